@@ -1,0 +1,808 @@
+// RRSPH_HIP: C-ABI implementation (host side) of the MI355X WCSPH step.
+//
+// The context owns the device buffers, one HIP stream at default priority
+// and (optionally) a captured hipGraph of one step.  Every entry point is
+// declared in include/rrsph_hip.h with the reference interface it replaces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rrsph_hip.h"
+#include "device_common.hpp"
+#include "diagnostics.hpp"
+#include "radix_sort.hpp"
+#include "step_kernels.hpp"
+
+using namespace rrsph;
+
+namespace {
+
+enum KernelClass : uint32_t {
+    K_PROLOGUE = 0,
+    K_UPSWEEP,
+    K_SCAN,
+    K_DOWNSWEEP,
+    K_REORDER,
+    K_DENSITY_SUM,
+    K_PAIR,
+    K_INTEGRATE,
+    K_NUM_CLASSES
+};
+const char* const kClassNames[K_NUM_CLASSES] = {"step_prologue", "radix_upsweep", "radix_scan", "radix_downsweep",
+                                                "reorder_kernel", "density_sum_kernel", "pair_kernel",
+                                                "integrate_kernel"};
+
+struct Profiler {
+    std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> rec;
+};
+
+template <class T> T* dalloc(size_t n, hipError_t& err) {
+    void* p = nullptr;
+    if (err == hipSuccess) err = hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T));
+    return (T*)p;
+}
+
+// Host copies of the smoothing-kernel W (SmoothingKernel.h:14-242) for constants.
+double host_w(const rrsph_params_c& p, double dist, int skf) {
+    const double h = p.hsml, pi = p.pi, q = dist / h;
+    switch (skf) {
+    case 2: return 1 / (pi * h * h) * std::exp(-q * q);
+    case 3: {
+        if (q > 2) return 0;
+        double t = 1 - 0.5 * q;
+        return 7 / (4 * pi * h * h) * (t * t * t * t) * (2 * q + 1);
+    }
+    case 4: return q <= 2 ? 5 / (16 * pi * h * h) * (2 - q) * (2 - q) * (2 - q) : 0;
+    default: {
+        double f = double(15) / (double(7) * pi * (h * h));
+        if (q <= 1) return f * (double(2) / double(3) - q * q + q * q * q * 0.5);
+        if (q <= 2) return f * (double(1) / double(6) * ((2 - q) * (2 - q) * (2 - q)));
+        return 0;
+    }
+    }
+}
+
+}  // namespace
+
+struct rrsph_ctx {
+    rrsph_params_c prm{};
+    StepParams P{};
+    std::string err;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool fp32 = true;
+    bool fused = true;
+    bool export_fields = false;
+    bool use_graph = true;
+    bool uploaded = false;
+    bool stepped = false;
+    uint32_t passes = 0;
+    uint32_t ntiles = 0;
+    // buffers
+    double2 *posA = nullptr, *posB = nullptr;
+    void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
+         *pred_scratch = nullptr;
+    uint32_t *oidA = nullptr, *oidB = nullptr;
+    uint32_t *key_next = nullptr, *kbuf[2] = {nullptr, nullptr}, *vbuf[2] = {nullptr, nullptr};
+    uint32_t* hist = nullptr;
+    uint32_t* LB = nullptr;
+    Ctrl* ctrl = nullptr;
+    Ctrl* util_ctrl = nullptr;
+    Ctrl* host_ctrl = nullptr;  // pinned mirror
+    double* red = nullptr;      // diagnostics partials
+    // staging for up/download (device)
+    double *st_r = nullptr, *st_v = nullptr, *st_rho = nullptr, *st_p = nullptr, *st_a = nullptr,
+           *st_drho = nullptr, *st_av = nullptr, *st_mu = nullptr;
+    int32_t* st_itype = nullptr;
+    uint32_t* st_u32 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    const uint32_t* sorted_keys = nullptr;
+    const uint32_t* sorted_perm = nullptr;
+};
+
+namespace {
+
+rrsph_status fail(rrsph_ctx* c, rrsph_status st, const std::string& msg) {
+    if (c) c->err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return fail(ctx, RRSPH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <class R> StateA<R> stateA(rrsph_ctx* c) {
+    return StateA<R>{c->posA, (real4<R>*)c->vaA, (real4<R>*)c->adA, c->oidA};
+}
+template <class R> StateB<R> stateB(rrsph_ctx* c, void* pred) {
+    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB};
+}
+
+uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
+
+// Launch helper that optionally records HIP events around the launch.
+struct Launcher {
+    rrsph_ctx* c;
+    Profiler* prof;
+    template <class F> void operator()(uint32_t cls, F&& launch) {
+        if (prof) {
+            hipEvent_t a, b;
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
+            launch();
+            (void)hipEventRecord(b, c->stream);
+            prof->rec.push_back({cls, {a, b}});
+        } else {
+            launch();
+        }
+    }
+};
+
+// Sort key_next (values = index) into kbuf/vbuf; returns final buffers.
+void enqueue_sort(rrsph_ctx* c, Ctrl* ctrl, Launcher& L) {
+    const uint32_t n = c->P.n;
+    const uint32_t* src_k = c->key_next;
+    const uint32_t* src_v = nullptr;
+    for (uint32_t p = 0; p < c->passes; ++p) {
+        uint32_t* dk = c->kbuf[p & 1];
+        uint32_t* dv = c->vbuf[p & 1];
+        const uint32_t shift = p * kRadixBits;
+        const uint32_t nt = c->ntiles;
+        L(K_UPSWEEP, [&] { radix_upsweep<<<nt, kBlock, 0, c->stream>>>(src_k, n, shift, c->hist, nt, ctrl); });
+        L(K_SCAN, [&] { radix_scan<<<1, 1024, 0, c->stream>>>(c->hist, kRadix * nt, ctrl); });
+        L(K_DOWNSWEEP, [&] {
+            radix_downsweep<<<nt, kBlock, 0, c->stream>>>(src_k, src_v, dk, dv, n, shift, c->hist, nt, ctrl);
+        });
+        src_k = dk;
+        src_v = dv;
+    }
+    c->sorted_keys = src_k;
+    c->sorted_perm = src_v;
+}
+
+template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
+    Launcher L{c, prof};
+    const StepParams& P = c->P;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
+    L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+    enqueue_sort(c, c->ctrl, L);
+    StateA<R> A = stateA<R>(c);
+    StateB<R> B = stateB<R>(c, c->pred);
+    L(K_REORDER, [&] {
+        reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB);
+    });
+    if (P.density_treatment == 0) {
+        L(K_DENSITY_SUM, [&] {
+            density_sum_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB);
+        });
+    }
+    real4<R>* der = (real4<R>*)c->der;
+    real4<R>* xs = (real4<R>*)c->xs;
+    if (c->fused) {
+        if (c->export_fields) {
+            L(K_PAIR, [&] {
+                pair_kernel<R, true, true><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A,
+                                                                            c->key_next, der, xs);
+            });
+        } else {
+            L(K_PAIR, [&] {
+                pair_kernel<R, true, false><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB,
+                                                                             A, c->key_next, der, xs);
+            });
+        }
+    } else {
+        L(K_PAIR, [&] {
+            pair_kernel<R, false, true><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A,
+                                                                         c->key_next, der, xs);
+        });
+        L(K_INTEGRATE, [&] {
+            integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next);
+        });
+    }
+}
+
+void enqueue_step(rrsph_ctx* c, Profiler* prof) {
+    if (c->fp32) enqueue_step_t<float>(c, prof);
+    else enqueue_step_t<double>(c, prof);
+}
+
+void drop_graph(rrsph_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
+rrsph_status build_graph(rrsph_ctx* ctx) {
+    drop_graph(ctx);
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(ctx, nullptr);
+    HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph));
+    HIP_TRY(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    return RRSPH_OK;
+}
+
+// Sort + reorder of the current state into B (with a scratch pred) for the
+// utility queries, using a control block that is never halted.
+template <class R> void enqueue_util_sort(rrsph_ctx* c) {
+    Launcher L{c, nullptr};
+    enqueue_sort(c, c->util_ctrl, L);
+    const uint32_t nb = grid_blocks(c->P.n, kBlock);
+    StateA<R> A = stateA<R>(c);
+    StateB<R> B = stateB<R>(c, c->pred_scratch);
+    reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB);
+}
+
+std::string status_message(uint32_t st) {
+    switch (st) {
+    case RRSPH_ERR_TOO_MANY_NEIGHBOURS: return "making neighbours grid: error occured";
+    case RRSPH_ERR_NEVER_ENDING: return "never ending simulation";
+    case RRSPH_ERR_OUTSIDE_BOUNDARIES: return "Encounter particle outside boundaries";
+    default: return "";
+    }
+}
+
+// Read the control block after the stream drained and turn it into a status.
+rrsph_status sync_status(rrsph_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const Ctrl& c = *ctx->host_ctrl;
+    if (c.halted && c.status != 0) {
+        return fail(ctx, (rrsph_status)c.status, status_message(c.status));
+    }
+    if (c.pending) {
+        if (ctx->P.consistency_check && c.nbr_overflow)
+            return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
+        if (ctx->P.consistency_check && c.outside > 0 && ctx->P.consistency_treatment == 1)
+            return fail(ctx, RRSPH_ERR_OUTSIDE_BOUNDARIES, status_message(RRSPH_ERR_OUTSIDE_BOUNDARIES));
+    }
+    return RRSPH_OK;
+}
+
+rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
+    StepParams& P = ctx->P;
+    P = StepParams{};
+    if (p.dim != 2) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: only dim = 2 is implemented");
+    if (p.ntotal == 0 || p.max_neighbours < 2 || !(p.hsml > 0) || !(p.cell_scale_k > 0))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: invalid sizes");
+    if (p.nfluid > p.ntotal) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: nfluid > ntotal");
+    P.n = p.ntotal;
+    P.nfluid = p.nfluid;
+    P.max_nbr = p.max_neighbours;
+    P.cell = p.cell_scale_k * p.hsml;
+    const double tx = (p.x_maxgeom - p.x_mingeom) / P.cell, ty = (p.y_maxgeom - p.y_mingeom) / P.cell;
+    if (!(tx >= 0) || !(ty >= 0) || tx > 65536.0 || ty > 65536.0)
+        return fail(ctx, RRSPH_ERR_TOO_MANY_CELLS, "can't make grid with so many cells");
+    P.ncx = (int32_t)(uint64_t)tx + 1;
+    P.ncy = (int32_t)(uint64_t)ty + 1;
+    const uint64_t ncells = (uint64_t)P.ncx * (uint64_t)P.ncy;
+    if (ncells + 2 >= (1ull << 31)) return fail(ctx, RRSPH_ERR_TOO_MANY_CELLS, "can't make grid with so many cells");
+    P.ncells = (uint32_t)ncells;
+    P.xmin = p.x_mingeom;
+    P.ymin = p.y_mingeom;
+    P.xmax = p.x_maxgeom;
+    P.ymax = p.y_maxgeom;
+    P.cut2 = P.cell * P.cell;
+    P.h = p.hsml;
+    P.mass = p.mass;
+    P.rho0 = p.rho0;
+    P.g = p.g;
+    const double h = p.hsml, pi = p.pi;
+    P.f_cubic = double(15) / (double(7) * pi * (h * h));
+    P.f_gauss = 1 / (pi * (h * h));
+    P.f_wend = 7 / (4 * pi * (h * h));
+    P.f_desb = 5 / (16 * pi * (h * h));
+    P.skf_density = p.density_skf;
+    P.skf_intf = p.intf_skf;
+    P.skf_av = p.artificial_viscosity_skf;
+    P.skf_xsph = p.average_velocity_skf;
+    P.skf_ap = p.artificial_pressure_skf;
+    P.eos_B = p.eos_c_sqr * p.rho0 / 7;
+    P.density_treatment = p.density_treatment;
+    P.dhc = p.density_delta_sph_coef * p.hsml * p.eos_c;
+    P.intf_approx = p.intf_sph_approximation;
+    P.art_pressure = p.artificial_pressure;
+    P.ap_coef = p.artificial_pressure_coef;
+    P.ap_index = p.artificial_pressure_index;
+    P.ap_wdelta = host_w(p, p.delta, p.artificial_pressure_skf);
+    P.av_on = p.artificial_viscosity;
+    P.av_alpha = p.artificial_shear_visc;
+    P.av_beta = p.artificial_bulk_visc;
+    P.av_c = p.eos_sound_vel;
+    const double etq = (double)0.1f;  // ArtificialViscosity.h:22 (a float literal)
+    P.av_eta2 = (p.hsml * etq) * (p.hsml * etq);
+    P.xsph_on = p.average_velocity;
+    P.xsph_eps = p.average_velocity_coef;
+    P.lj_on = p.boundary_treatment == 1;
+    P.lj_r0 = 2 * p.hsml;
+    P.lj_D = 5 * p.g * p.depth;
+    P.dt_method = p.dt_correction_method;
+    P.dt_dynamic_mode = p.dt_dynamic_mode;
+    P.dt0 = p.dt;
+    P.cfl = p.CFL_coef;
+    P.c0 = p.eos_c;
+    P.never_ending = (p.hsml / p.eos_c) * 1.E-6;
+    P.consistency_check = p.consistency_check;
+    P.consistency_treatment = p.consistency_treatment;
+    P.boundary_motion = p.boundary_motion;
+    P.nwm = p.nwm;
+    P.nwm_axis = -1;
+    if (p.nwm == 2 || p.nwm == 3 || p.nwm == 6) {
+        if (p.nwm_direction == 1) P.nwm_axis = 0;
+        else if (p.nwm_direction == 2) P.nwm_axis = 1;
+    } else if (p.nwm == 5) {
+        P.nwm_axis = 0;
+    }
+    bool range_ok = !(p.nwm_particles_start < p.nfluid || p.nwm_particles_end < p.nfluid ||
+                      p.nwm_particles_end < p.nwm_particles_start || p.nwm_particles_end > p.ntotal);
+    if (!range_ok) P.nwm = 0;  // CLTimeIntegration.cpp:375-385: wrong range -> no wave maker
+    P.nwm_start = p.nwm_particles_start;
+    P.nwm_end = p.nwm_particles_end;
+    P.nwm_time_start = p.nwm_time_start;
+    P.nwm_freq = p.nwm_freq;
+    P.nwm_delta = p.nwm_phase - p.nwm_freq * p.nwm_time_start;
+    {
+        const double kd = p.nwm_wave_number * p.depth, H = p.nwm_wave_magnitude;
+        const double sh = std::sinh(kd), ch = std::cosh(kd);
+        const double m1 = 2 * sh * sh / (sh * ch + kd);
+        P.nwm_S0 = m1 != 0 ? H / m1 : 0;
+        P.nwm_e2 = (m1 != 0 && sh != 0) ? (H * H / (32 * p.depth)) * (3 * ch / (sh * sh * sh) - 2 / m1) : 0;
+        const double Hs = p.nwm_wave_magnitude, hd = p.depth;
+        P.sol_H = Hs;
+        P.sol_h = hd;
+        P.sol_k = std::sqrt(3 * Hs / (4 * hd * hd * (Hs + hd)));
+        P.sol_c = std::sqrt(p.g * (Hs + hd));
+        P.sol_Tf = 2 / (P.sol_k * P.sol_c) * (3.8 + Hs / hd);
+    }
+    if (p.piston_nharm < 0 || p.piston_nharm > 32)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: piston_nharm must be in [0, 32]");
+    P.piston_nharm = p.piston_nharm;
+    P.piston_omega = p.piston_omega;
+    for (int k = 0; k < 32; ++k) {
+        P.piston_a[k] = p.piston_a[k];
+        P.piston_b[k] = p.piston_b[k];
+    }
+    P.nblocks = grid_blocks(P.n, kBlock);
+    P.blocks_per_xcd = (P.nblocks + kNumXcd - 1) / kNumXcd;
+    // radix passes over the key bits (dead key = ncells must fit)
+    uint32_t bits = 0;
+    while ((1ull << bits) <= (uint64_t)P.ncells) ++bits;
+    ctx->passes = (bits + kRadixBits - 1) / kRadixBits;
+    if (ctx->passes == 0) ctx->passes = 1;
+    ctx->ntiles = grid_blocks(P.n, kSortTile);
+    ctx->fp32 = p.precision == RRSPH_FP32;
+    ctx->fused = !(p.dt_correction_method == 2 && p.dt_dynamic_mode == 1);
+    return RRSPH_OK;
+}
+
+void free_all(rrsph_ctx* c) {
+    drop_graph(c);
+    void* ptrs[] = {c->posA, c->posB, c->vaA, c->vaB, c->adA, c->pred, c->der, c->xs, c->pred_scratch, c->oidA,
+                    c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->LB, c->ctrl,
+                    c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
+                    c->st_mu, c->st_itype, c->st_u32};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
+    if (!params || !out) return RRSPH_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    rrsph_ctx* ctx = new rrsph_ctx();
+    if (params->struct_size != sizeof(rrsph_params_c)) {
+        ctx->err = "RRSPH_HIP: rrsph_params_c size mismatch";
+        *out = ctx;
+        return RRSPH_ERR_INVALID_ARGUMENT;
+    }
+    ctx->prm = *params;
+    *out = ctx;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(ctx, RRSPH_ERR_NO_DEVICE, "RRSPH_HIP: no HIP device");
+    rrsph_status st = setup_params(ctx, *params);
+    if (st != RRSPH_OK) return st;
+    HIP_TRY(hipGetDevice(&ctx->device));
+    HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->own_stream = true;
+    const size_t n = ctx->P.n;
+    const size_t r4 = ctx->fp32 ? sizeof(float4) : sizeof(double4);
+    hipError_t e = hipSuccess;
+    ctx->posA = dalloc<double2>(n, e);
+    ctx->posB = dalloc<double2>(n, e);
+    ctx->vaA = dalloc<char>(n * r4, e);
+    ctx->vaB = dalloc<char>(n * r4, e);
+    ctx->adA = dalloc<char>(n * r4, e);
+    ctx->pred = dalloc<char>(n * r4, e);
+    ctx->der = dalloc<char>(n * r4, e);
+    ctx->xs = dalloc<char>(n * r4, e);
+    ctx->pred_scratch = dalloc<char>(n * r4, e);
+    ctx->oidA = dalloc<uint32_t>(n, e);
+    ctx->oidB = dalloc<uint32_t>(n, e);
+    ctx->key_next = dalloc<uint32_t>(n, e);
+    ctx->kbuf[0] = dalloc<uint32_t>(n, e);
+    ctx->kbuf[1] = dalloc<uint32_t>(n, e);
+    ctx->vbuf[0] = dalloc<uint32_t>(n, e);
+    ctx->vbuf[1] = dalloc<uint32_t>(n, e);
+    ctx->hist = dalloc<uint32_t>((size_t)kRadix * ctx->ntiles, e);
+    ctx->LB = dalloc<uint32_t>((size_t)ctx->P.ncells + 2, e);
+    ctx->ctrl = dalloc<Ctrl>(1, e);
+    ctx->util_ctrl = dalloc<Ctrl>(1, e);
+    ctx->red = dalloc<double>(kDiagSlots * (size_t)kDiagMaxBlocks + 64, e);
+    if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    std::memset(ctx->host_ctrl, 0, sizeof(Ctrl));
+    HIP_TRY(hipMemcpy(ctx->util_ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
+    return RRSPH_OK;
+}
+
+void rrsph_hip_destroy(rrsph_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_all(ctx);
+    delete ctx;
+}
+
+const char* rrsph_hip_last_error(const rrsph_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static rrsph_status ensure_staging(rrsph_ctx* ctx) {
+    if (ctx->st_r) return RRSPH_OK;
+    const size_t n = ctx->P.n;
+    hipError_t e = hipSuccess;
+    ctx->st_r = dalloc<double>(2 * n, e);
+    ctx->st_v = dalloc<double>(2 * n, e);
+    ctx->st_rho = dalloc<double>(n, e);
+    ctx->st_p = dalloc<double>(n, e);
+    ctx->st_a = dalloc<double>(2 * n, e);
+    ctx->st_drho = dalloc<double>(n, e);
+    ctx->st_av = dalloc<double>(2 * n, e);
+    ctx->st_mu = dalloc<double>(n, e);
+    ctx->st_itype = dalloc<int32_t>(n, e);
+    ctx->st_u32 = dalloc<uint32_t>(n, e);
+    if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, const double* rho,
+                              const int32_t* itype, uint32_t ntotal) {
+    if (!ctx || !ctx->stream) return RRSPH_ERR_INVALID_ARGUMENT;
+    if (!r || !v || !rho || !itype || ntotal != ctx->P.n)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "dump corrupted: dump ntotal doesn't match params.ntotal!");
+    rrsph_status st = ensure_staging(ctx);
+    if (st != RRSPH_OK) return st;
+    const size_t n = ctx->P.n;
+    HIP_TRY(hipMemcpyAsync(ctx->st_r, r, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->st_v, v, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->st_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->st_itype, itype, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    Ctrl c{};
+    c.time = ctx->prm.start_time;
+    c.dt = ctx->prm.dt;
+    c.t_end = INFINITY;
+    *ctx->host_ctrl = c;
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    if (ctx->fp32)
+        init_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho, ctx->st_itype,
+                                                           stateA<float>(ctx), ctx->key_next, ctx->ctrl);
+    else
+        init_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho,
+                                                            ctx->st_itype, stateA<double>(ctx), ctx->key_next,
+                                                            ctx->ctrl);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->uploaded = true;
+    ctx->stepped = false;
+    return RRSPH_OK;
+}
+
+static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
+    if (ctx->use_graph) {
+        if (!ctx->gexec) {
+            rrsph_status st = build_graph(ctx);
+            if (st != RRSPH_OK) return st;
+        }
+        for (uint32_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
+    } else {
+        for (uint32_t k = 0; k < nsteps; ++k) enqueue_step(ctx, nullptr);
+        HIP_TRY(hipGetLastError());
+    }
+    if (nsteps) ctx->stepped = true;
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_step(rrsph_ctx* ctx, uint32_t nsteps) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    rrsph_status st = launch_steps(ctx, nsteps);
+    if (st != RRSPH_OK) return st;
+    return sync_status(ctx);
+}
+
+rrsph_status rrsph_hip_run_until(rrsph_ctx* ctx, double t_end, uint64_t max_steps, uint64_t* steps_taken) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    // set t_end on the device (the prologue halts with status 0 past it)
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint64_t step0 = ctx->host_ctrl->step + (ctx->host_ctrl->pending ? 1 : 0);
+    if (ctx->host_ctrl->halted && ctx->host_ctrl->status == 0) ctx->host_ctrl->halted = 0;
+    ctx->host_ctrl->t_end = t_end;
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    uint64_t done = 0;
+    rrsph_status st = RRSPH_OK;
+    while (done < max_steps) {
+        const uint32_t chunk = (uint32_t)std::min<uint64_t>(max_steps - done, 256);
+        st = launch_steps(ctx, chunk);
+        if (st != RRSPH_OK) break;
+        st = sync_status(ctx);
+        done += chunk;
+        if (st != RRSPH_OK || ctx->host_ctrl->halted) break;
+    }
+    const uint64_t step1 = ctx->host_ctrl->step + (ctx->host_ctrl->pending ? 1 : 0);
+    if (steps_taken) *steps_taken = step1 - step0;
+    // clear the stop so later calls run
+    ctx->host_ctrl->t_end = INFINITY;
+    if (ctx->host_ctrl->halted && ctx->host_ctrl->status == 0) ctx->host_ctrl->halted = 0;
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return st;
+}
+
+rrsph_status rrsph_hip_get_time(rrsph_ctx* ctx, double* time, double* dt, uint64_t* step) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const Ctrl& c = *ctx->host_ctrl;
+    if (time) *time = c.pending ? c.time + c.dt_new : c.time;
+    if (dt) *dt = c.pending ? c.dt_new : c.dt;
+    if (step) *step = c.step + (c.pending ? 1 : 0);
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, double* v, double* rho, double* p,
+                                int32_t* itype) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    const size_t n = ctx->P.n;
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    if (ctx->fp32)
+        download_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(
+            ctx->P, stateA<float>(ctx), ctx->stepped ? (const float4*)ctx->pred : nullptr, ctx->st_r, ctx->st_v,
+            ctx->st_rho, ctx->st_p, ctx->st_itype);
+    else
+        download_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(
+            ctx->P, stateA<double>(ctx), ctx->stepped ? (const double4*)ctx->pred : nullptr, ctx->st_r, ctx->st_v,
+            ctx->st_rho, ctx->st_p, ctx->st_itype);
+    HIP_TRY(hipGetLastError());
+    if ((fields & RRSPH_FIELD_R) && r)
+        HIP_TRY(hipMemcpyAsync(r, ctx->st_r, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((fields & RRSPH_FIELD_V) && v)
+        HIP_TRY(hipMemcpyAsync(v, ctx->st_v, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((fields & RRSPH_FIELD_RHO) && rho)
+        HIP_TRY(hipMemcpyAsync(rho, ctx->st_rho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((fields & RRSPH_FIELD_P) && p)
+        HIP_TRY(hipMemcpyAsync(p, ctx->st_p, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if ((fields & RRSPH_FIELD_ITYPE) && itype)
+        HIP_TRY(hipMemcpyAsync(itype, ctx->st_itype, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_set_export_fields(rrsph_ctx* ctx, int32_t enable) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    if ((bool)enable != ctx->export_fields) {
+        ctx->export_fields = enable != 0;
+        drop_graph(ctx);
+    }
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_fields(rrsph_ctx* ctx, double* a, double* drho, double* av, double* arvmu) {
+    if (!ctx || !ctx->stepped) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no step taken yet");
+    if (ctx->fused && !ctx->export_fields)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: enable rrsph_hip_set_export_fields before stepping");
+    const size_t n = ctx->P.n;
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    if (ctx->fp32)
+        fields_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->oidA, (const float4*)ctx->der,
+                                                             (const float4*)ctx->xs, ctx->st_a, ctx->st_drho,
+                                                             ctx->st_av, ctx->st_mu);
+    else
+        fields_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->oidA, (const double4*)ctx->der,
+                                                              (const double4*)ctx->xs, ctx->st_a, ctx->st_drho,
+                                                              ctx->st_av, ctx->st_mu);
+    HIP_TRY(hipGetLastError());
+    if (a) HIP_TRY(hipMemcpyAsync(a, ctx->st_a, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (drho) HIP_TRY(hipMemcpyAsync(drho, ctx->st_drho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (av) HIP_TRY(hipMemcpyAsync(av, ctx->st_av, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (arvmu) HIP_TRY(hipMemcpyAsync(arvmu, ctx->st_mu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_cell_keys(rrsph_ctx* ctx, uint32_t* keys) {
+    if (!ctx || !ctx->uploaded || !keys) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    morton_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, ctx->oidA, ctx->st_u32);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(keys, ctx->st_u32, ctx->P.n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_neighbours(rrsph_ctx* ctx, uint32_t* out, uint32_t max_nbr) {
+    if (!ctx || !ctx->uploaded || !out || max_nbr < 2)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    const size_t n = ctx->P.n;
+    uint32_t* dout = nullptr;
+    HIP_TRY(hipMalloc((void**)&dout, n * (size_t)max_nbr * sizeof(uint32_t)));
+    Ctrl zero{};
+    HIP_TRY(hipMemcpyAsync(ctx->util_ctrl, &zero, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    if (ctx->fp32) {
+        enqueue_util_sort<float>(ctx);
+        neighbours_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<float>(ctx, ctx->pred_scratch),
+                                                                 ctx->sorted_keys, ctx->LB, dout, max_nbr,
+                                                                 ctx->util_ctrl);
+    } else {
+        enqueue_util_sort<double>(ctx);
+        neighbours_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<double>(ctx, ctx->pred_scratch),
+                                                                  ctx->sorted_keys, ctx->LB, dout, max_nbr,
+                                                                  ctx->util_ctrl);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout, n * (size_t)max_nbr * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->util_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipFree(dout));
+    const uint32_t overflow = ctx->host_ctrl->nbr_overflow;
+    // restore the main control block mirror
+    HIP_TRY(hipMemcpy(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+    if (overflow && ctx->P.consistency_check)
+        return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_diagnostics(rrsph_ctx* ctx, rrsph_diagnostics* out) {
+    if (!ctx || !ctx->uploaded || !out) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    const uint32_t nblk = std::min<uint32_t>(kDiagMaxBlocks, grid_blocks(ctx->P.n, kBlock));
+    if (ctx->fp32)
+        diag_partials<float><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA<float>(ctx), ctx->red);
+    else
+        diag_partials<double><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA<double>(ctx), ctx->red);
+    diag_final<<<1, kBlock, 0, ctx->stream>>>(ctx->red, nblk);
+    HIP_TRY(hipGetLastError());
+    double h[kDiagSlots];
+    HIP_TRY(hipMemcpyAsync(h, ctx->red + (size_t)kDiagSlots * kDiagMaxBlocks, sizeof(h), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    double t, dt;
+    uint64_t step;
+    rrsph_status st = rrsph_hip_get_time(ctx, &t, &dt, &step);
+    if (st != RRSPH_OK) return st;
+    out->time = t;
+    out->dt = dt;
+    out->step = step;
+    out->n_fluid_alive = (uint32_t)h[0];
+    out->n_outside = (uint32_t)h[1];
+    out->mass_fluid = h[0] * ctx->P.mass;
+    out->kinetic_energy = h[2];
+    out->potential_energy = h[3];
+    out->internal_energy = h[4];
+    out->max_speed = h[5];
+    out->rho_min = -h[6];
+    out->rho_max = h[7];
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t ngauges, double search_n,
+                              int32_t ptype, double* out) {
+    if (!ctx || !ctx->uploaded || (!x_gauges && ngauges) || (!out && ngauges))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    if (ngauges == 0) return RRSPH_OK;
+    if (ngauges > 4096) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: at most 4096 gauges");
+    double* dx = ctx->red;  // ngauges inputs followed by ngauges results
+    HIP_TRY(hipMemcpyAsync(dx, x_gauges, ngauges * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemsetAsync(dx + ngauges, 0, ngauges * sizeof(double), ctx->stream));
+    const double radius = search_n * ctx->P.h;
+    const uint32_t nblk = std::min<uint32_t>(1024, grid_blocks(ctx->P.n, kBlock));
+    dim3 grid(nblk, ngauges);
+    if (ctx->fp32)
+        gauge_kernel<float><<<grid, kBlock, 0, ctx->stream>>>(ctx->P.n, stateA<float>(ctx), dx, radius, ptype,
+                                                              (unsigned long long*)(dx + ngauges));
+    else
+        gauge_kernel<double><<<grid, kBlock, 0, ctx->stream>>>(ctx->P.n, stateA<double>(ctx), dx, radius, ptype,
+                                                               (unsigned long long*)(dx + ngauges));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dx + ngauges, ngauges * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
+void* rrsph_hip_stream(rrsph_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+rrsph_status rrsph_hip_set_graph(rrsph_ctx* ctx, int32_t enable) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    ctx->use_graph = enable != 0;
+    if (!ctx->use_graph) drop_graph(ctx);
+    return RRSPH_OK;
+}
+
+uint32_t rrsph_hip_kernel_class_count(void) { return K_NUM_CLASSES; }
+const char* rrsph_hip_kernel_class_name(uint32_t k) { return k < K_NUM_CLASSES ? kClassNames[k] : ""; }
+
+rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* avg_ms, uint32_t n_classes,
+                                     double* total_ms) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    Profiler prof;
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, ctx->stream));
+    for (uint32_t k = 0; k < nsteps; ++k) enqueue_step(ctx, &prof);
+    HIP_TRY(hipEventRecord(t1, ctx->stream));
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (nsteps) ctx->stepped = true;
+    std::vector<double> sum(K_NUM_CLASSES, 0.0);
+    std::vector<uint32_t> cnt(K_NUM_CLASSES, 0);
+    for (auto& r : prof.rec) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, r.second.first, r.second.second);
+        sum[r.first] += ms;
+        cnt[r.first] += 1;
+        (void)hipEventDestroy(r.second.first);
+        (void)hipEventDestroy(r.second.second);
+    }
+    for (uint32_t k = 0; k < n_classes && k < K_NUM_CLASSES; ++k) avg_ms[k] = cnt[k] ? sum[k] / cnt[k] : 0.0;
+    float tot = 0;
+    (void)hipEventElapsedTime(&tot, t0, t1);
+    if (total_ms) *total_ms = tot;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    return sync_status(ctx);
+}
+
+rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total_ms) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    if (ctx->use_graph && !ctx->gexec) {
+        rrsph_status st = build_graph(ctx);
+        if (st != RRSPH_OK) return st;
+    }
+    hipEvent_t t0, t1;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    HIP_TRY(hipEventRecord(t0, ctx->stream));
+    rrsph_status st = launch_steps(ctx, nsteps);
+    HIP_TRY(hipEventRecord(t1, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, t0, t1);
+    if (total_ms) *total_ms = ms;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (st != RRSPH_OK) return st;
+    return sync_status(ctx);
+}
+
+uint32_t RRSPH_HIP_GetSpecificVersionMajor(void) { return RRSPH_HIP_VERSION_MAJOR; }
+uint32_t RRSPH_HIP_GetSpecificVersionMinor(void) { return RRSPH_HIP_VERSION_MINOR; }
+uint32_t RRSPH_HIP_GetSpecificVersionPatch(void) { return RRSPH_HIP_VERSION_PATCH; }
+const char* RRSPH_HIP_GetSpecificVersionName(void) { return "RRSPH_HIP"; }
+
+}  // extern "C"

@@ -1,0 +1,589 @@
+// The WCSPH step as HIP kernels for gfx950 (2D).
+//
+// Device layout (all arrays in the particle order of the last cell sort):
+//   pos   double2   position (FP64 in both precisions)
+//   va    real4     (vx, vy, rho, itype)
+//   ad    real4     (ax, ay, drho, 0) of the previous step (predictor input)
+//   oid   uint32    original particle index
+//   pred  real4     (vx^, vy^, rho^, p): predicted velocity and density, pressure
+// Two copies of (pos, va, oid) ping-pong: "A" holds the state between steps,
+// "B" the freshly sorted copy a step reads from.
+//
+// Step = prologue -> radix sort of the cell keys -> reorder (gather A -> B,
+// predict, pressure, cell-start table) -> [density summation pass] -> pair
+// pass (continuity + forces, and whole_step fused in when dt is known before
+// the step) -> [integrate, when dt comes from this step's reductions].
+#pragma once
+
+#include "device_common.hpp"
+#include "physics.hpp"
+
+namespace rrsph {
+
+template <class R> struct StateA {
+    double2* pos;
+    real4<R>* va;
+    real4<R>* ad;
+    uint32_t* oid;
+};
+template <class R> struct StateB {
+    double2* pos;
+    real4<R>* va;
+    real4<R>* pred;
+    uint32_t* oid;
+};
+
+// ------------------------------------------------------------------ grid keys
+// Cell coordinate by truncation of (r - r_min) / cell, as get_cell_idx2
+// (GridUtils.h:135-144); the linear key is x-major (cx * ncy + cy) so the
+// three cells of one grid column are contiguous after sorting and are walked
+// in the reference's x-major neighbour-cell order (GridUtils.h:209-214).
+__device__ __forceinline__ int cell_coord(double r, double rmin, double cell, int ncell, uint32_t& bad) {
+    const double t = (r - rmin) / cell;
+    if (t >= 0.0 && t < (double)ncell) return (int)t;
+    bad = 1;
+    return t >= (double)ncell ? ncell - 1 : 0;  // NaN and negative -> 0
+}
+
+__device__ __forceinline__ uint32_t cell_key(const StepParams& P, double x, double y, uint32_t& bad) {
+    const int cx = cell_coord(x, P.xmin, P.cell, P.ncx, bad);
+    const int cy = cell_coord(y, P.ymin, P.cell, P.ncy, bad);
+    return (uint32_t)cx * (uint32_t)P.ncy + (uint32_t)cy;
+}
+
+__device__ __forceinline__ uint32_t split2(uint32_t x) {
+    x &= 0x0000FFFFu;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
+// ------------------------------------------------------------------ prologue
+__device__ double piston_velocity(const StepParams& P, double t) {
+    if (P.nwm == 6) {
+        double v = 0;
+        for (int k = 1; k <= P.piston_nharm; ++k) {
+            const double kw = k * P.piston_omega;
+            v += kw * (P.piston_a[k - 1] * cos(kw * t) - P.piston_b[k - 1] * sin(kw * t));
+        }
+        return v;
+    }
+    if (P.nwm == 5) {  // WaveMaker.cpp:105-128
+        const double u = tanh(P.sol_k * P.sol_c * (t - P.sol_Tf));
+        const double H = P.sol_H, h = P.sol_h, k = P.sol_k;
+        const double top = k * H * (h + H - H * u * u) + 2 * k * (H * u) * (H * u);
+        const double bottom = (k * (h + H - H * u * u)) * (k * (h + H - H * u * u));
+        return top / bottom * (1 - u * u) * (k * P.sol_c);
+    }
+    // cl/TimeIntegration.cl:206-229 (first order + product-form second order)
+    double vx = 0.5 * P.nwm_S0 * P.nwm_freq * cos(P.nwm_freq * t + P.nwm_delta);
+    if (P.nwm == 3) vx += 2 * P.nwm_freq * P.nwm_e2 * cos(2 * P.nwm_freq * t + 2 * P.nwm_delta);
+    return vx;
+}
+
+__device__ __forceinline__ double dbits(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// One thread: commit the previous step, decide whether to halt, snapshot.
+__global__ void step_prologue(StepParams P, Ctrl* ctrl) {
+    Ctrl& c = *ctrl;
+    if (c.halted) return;
+    if (c.pending) {
+        c.time += c.dt_new;
+        c.dt = c.dt_new;
+        c.step += 1;
+        c.pending = 0;
+        if (P.consistency_check && c.nbr_overflow) {
+            c.halted = 1;
+            c.status = 3;  // RRSPH_ERR_TOO_MANY_NEIGHBOURS
+            return;
+        }
+        if (P.consistency_check && c.outside > 0 && P.consistency_treatment == 1) {
+            c.halted = 1;
+            c.status = 5;  // RRSPH_ERR_OUTSIDE_BOUNDARIES
+            return;
+        }
+    }
+    if (c.time > c.t_end) {  // loop condition time <= simulation_time
+        c.halted = 1;
+        c.status = 0;
+        return;
+    }
+    c.cur_time = c.time;
+    c.cur_dt = c.dt;
+    c.cur_first = c.step == 0;
+    // dt of whole_step when it does not depend on this step's reductions
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 0) {
+        // RRSPH_OMP OpenMP build: max |a| and max mu stay 0 (UpdateAcceleration.cpp:107-124)
+        c.cur_dt_const = P.cfl * fmin(sqrt(P.h / 0.0), P.h / (P.c0 + 0.0));
+    } else {
+        c.cur_dt_const = c.dt;
+    }
+    const bool nwm_on = P.boundary_motion && P.nwm != 0 && c.time >= P.nwm_time_start;
+    c.piston_active = nwm_on && P.nwm != 4 && P.nwm_axis >= 0;
+    c.piston_v = c.piston_active ? piston_velocity(P, c.time) : 0.0;
+    c.wall_disappear = nwm_on && P.nwm == 4 && !c.nwm_done;
+    if (c.wall_disappear) c.nwm_done = 1;
+    c.amax_bits = 0;
+    c.mumax_bits = 0;
+    c.outside = 0;
+}
+
+// ------------------------------------------------------------------ reorder
+// Gather the state into sorted order, predict (PredictHalfStep.cpp:16-22),
+// compute the pressure of the current density (Density.h:112) and fill the
+// cell-start table: LB[c] = first sorted index whose key >= c.
+template <class R>
+__global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
+                                                         const uint32_t* __restrict__ skey,
+                                                         const uint32_t* __restrict__ perm, StateA<R> A, StateB<R> B,
+                                                         uint32_t* __restrict__ LB) {
+    if (ctrl->halted) return;
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const uint32_t k = skey[s];
+    const uint32_t src = perm[s];
+    const double2 pos = A.pos[src];
+    const real4<R> va = A.va[src];
+    const real4<R> ad = A.ad[src];
+    const uint32_t oid = A.oid[src];
+    B.pos[s] = pos;
+    B.va[s] = va;
+    B.oid[s] = oid;
+    const R dt = (R)ctrl->cur_dt;
+    const bool cont = P.density_treatment != 0;
+    real4<R> pr;
+    pr.x = va.x + ad.x * dt * R(0.5);
+    pr.y = va.y + ad.y * dt * R(0.5);
+    pr.z = cont ? va.z + ad.z * dt * R(0.5) : va.z;
+    pr.w = eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
+    B.pred[s] = pr;
+    // cell-start table over keys 0 .. ncells (+1 sentinel entry)
+    const uint32_t prev = s == 0 ? 0u : skey[s - 1] + 1u;
+    for (uint32_t c = prev; c <= k; ++c) LB[c] = s;
+    if (s == P.n - 1)
+        for (uint32_t c = k + 1; c <= P.ncells + 1; ++c) LB[c] = P.n;
+}
+
+// ------------------------------------------------------------------ neighbour walk
+// Calls f(i, dx, dy, d2) for every neighbour i != s of particle s with
+// |r_i - r_s|^2 < (k h)^2 decided in FP64 without contraction, in the
+// reference's cell order; stops adding after max_nbr - 1 neighbours and
+// reports the overflow (GridFind.cpp:144-176).
+template <class F>
+__device__ __forceinline__ void for_each_neighbour(const StepParams& P, uint32_t cap, uint32_t s, uint32_t key,
+                                                   double2 pj, const double2* __restrict__ pos,
+                                                   const uint32_t* __restrict__ LB, bool& overflow, F&& f) {
+    const int cx = (int)(key / (uint32_t)P.ncy);
+    const int cy = (int)key - cx * P.ncy;
+    const int y0 = cy > 0 ? cy - 1 : 0;
+    const int y1 = cy + 1 < P.ncy ? cy + 1 : P.ncy - 1;
+    uint32_t count = 0;
+#pragma unroll 1
+    for (int dc = -1; dc <= 1; ++dc) {
+        const int col = cx + dc;
+        if (col < 0 || col >= P.ncx) continue;
+        const uint32_t b = LB[col * P.ncy + y0];
+        const uint32_t e = LB[col * P.ncy + y1 + 1];
+#pragma unroll 1
+        for (uint32_t i = b; i < e; ++i) {
+            if (i == s) continue;
+            const double2 pi = pos[i];
+            const double dx = __dsub_rn(pi.x, pj.x);
+            const double dy = __dsub_rn(pi.y, pj.y);
+            const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+            if (!(d2 < P.cut2)) continue;
+            if (count == cap) {
+                overflow = true;
+                continue;
+            }
+            ++count;
+            f(i, dx, dy, d2);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ summation density
+// Density.h:25-61: rho = m W(0) + sum m W_ij, p = EOS(rho); written into the
+// predicted slots (forces use this rho) of pred.
+template <class R>
+__global__ void __launch_bounds__(kBlock) density_sum_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
+                                                             const uint32_t* __restrict__ skey,
+                                                             const uint32_t* __restrict__ LB) {
+    if (ctrl->halted) return;
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const KernelConsts<R> K = kernel_consts<R>(P);
+    const R m = (R)P.mass;
+    const real4<R> va = B.va[s];
+    R rho = m * kernel_w<R>(K, P.skf_density, R(0));
+    bool overflow = false;
+    if ((int)va.w != 0) {
+        const double2 pj = B.pos[s];
+        for_each_neighbour(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, [&](uint32_t, double, double, double d2) {
+            rho += m * kernel_w<R>(K, P.skf_density, (R)sqrt(d2));
+        });
+    }
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+    real4<R> pr = B.pred[s];
+    pr.z = rho;
+    pr.w = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
+    B.pred[s] = pr;
+}
+
+// ------------------------------------------------------------------ whole_step
+// WholeStep.cpp:19-61 for one particle, with the RRSPH_CL boundary motion
+// (cl/TimeIntegration.cl:190-193) and wave maker (:196-255) when enabled.
+template <class R>
+__device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c, uint32_t s, double2 pos,
+                                              real4<R> va, const real4<R>& pred, uint32_t oid, R ax, R ay, R drho,
+                                              R avx, R avy, double dt, StateA<R> A, uint32_t* __restrict__ key_next,
+                                              uint32_t& outside, uint32_t& bad) {
+    const double vdt_d = c.cur_first ? dt * 0.5 : dt;
+    const R vdt = (R)vdt_d;
+    int itype = (int)va.w;
+    R rho = P.density_treatment != 0 ? va.z + drho * vdt : pred.z;
+    R vx = va.x, vy = va.y;
+    if (c.wall_disappear && oid >= P.nwm_start && oid < P.nwm_end && oid >= P.nfluid) itype = 0;
+    if (itype > 0) {
+        vx += ax * vdt;
+        vy += ay * vdt;
+        if (P.xsph_on) {
+            vx += avx;
+            vy += avy;
+        }
+        const double nx = pos.x + (double)vx * dt;
+        const double ny = pos.y + (double)vy * dt;
+        if (P.consistency_treatment == 2) {
+            if (nx < P.xmax && nx > P.xmin && ny < P.ymax && ny > P.ymin) {
+                pos.x = nx;
+                pos.y = ny;
+            } else {
+                itype = 0;
+            }
+        } else {
+            pos.x = nx;
+            pos.y = ny;
+        }
+    } else if (P.boundary_motion && itype < 0) {
+        if (c.piston_active && oid >= P.nwm_start && oid < P.nwm_end && oid >= P.nfluid) {
+            vx = P.nwm_axis == 0 ? (R)c.piston_v : R(0);
+            vy = P.nwm_axis == 1 ? (R)c.piston_v : R(0);
+        }
+        pos.x += (double)vx * dt;
+        pos.y += (double)vy * dt;
+    }
+    const bool inside = pos.x < P.xmax && pos.x > P.xmin && pos.y < P.ymax && pos.y > P.ymin;
+    if (!inside) outside += 1;  // NaN positions fail the comparisons too
+    A.pos[s] = pos;
+    real4<R> nva;
+    nva.x = vx;
+    nva.y = vy;
+    nva.z = rho;
+    nva.w = (R)itype;
+    A.va[s] = nva;
+    real4<R> nad;
+    nad.x = ax;
+    nad.y = ay;
+    nad.z = drho;
+    nad.w = R(0);
+    A.ad[s] = nad;
+    A.oid[s] = oid;
+    key_next[s] = itype == 0 ? P.ncells : cell_key(P, pos.x, pos.y, bad);
+}
+
+__device__ __forceinline__ void wave_count_add(uint32_t* dst, uint32_t v) {
+    // one atomic per wave
+    const unsigned long long any = __ballot(v != 0);
+    if (any == 0) return;
+    uint32_t sum = v;
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (__lane_id() == 0) atomicAdd(dst, sum);
+}
+
+template <class R> __device__ __forceinline__ void wave_max_bits(unsigned long long* dst, R v) {
+    double m = (double)v;
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (__lane_id() == 0 && m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+}
+
+// ------------------------------------------------------------------ pair pass
+// Per particle j (sorted index s): continuity density rate (Density.h:88-113),
+// acceleration = gravity + LJ + pressure + artificial viscosity
+// (UpdateAcceleration.cpp:59-100) and XSPH, in one walk over the neighbours.
+// FUSED: whole_step runs in the epilogue (dt known before the step).
+// EXPORT: also store a, drho, av, mu in sorted order for download.
+template <class R, bool FUSED, bool EXPORT>
+__global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
+                                                      const uint32_t* __restrict__ skey,
+                                                      const uint32_t* __restrict__ LB, StateA<R> A,
+                                                      uint32_t* __restrict__ key_next, real4<R>* __restrict__ der,
+                                                      real4<R>* __restrict__ xs) {
+    if (ctrl->halted) return;
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    const bool live = s < P.n;
+    const KernelConsts<R> K = kernel_consts<R>(P);
+    const R m = (R)P.mass;
+    R ax = R(0), ay = -(R)P.g, drho = R(0), avx = R(0), avy = R(0), amu = R(0);
+    bool overflow = false;
+    double2 pj = make_double2(0, 0);
+    real4<R> vaj{}, prj{};
+    uint32_t oid = 0;
+    if (live) {
+        pj = B.pos[s];
+        vaj = B.va[s];
+        prj = B.pred[s];
+        oid = B.oid[s];
+    }
+    const int tj = (int)vaj.w;
+    if (live && tj != 0) {
+        const bool cont = P.density_treatment != 0;
+        const bool delta = P.density_treatment == 2;
+        const R h = K.h;
+        const R rhoj = vaj.z, rhohj = prj.z, pjres = prj.w;
+        const R pj_rho2 = pjres / (rhohj * rhohj);
+        const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, cs = (R)P.av_c, eta2 = (R)P.av_eta2;
+        const R two_eps = R(2) * (R)P.xsph_eps;
+        const R lj_r0 = (R)P.lj_r0, lj_D = (R)P.lj_D;
+        const R dhc2m = R(2) * (R)P.dhc * m;
+        for_each_neighbour(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
+                           [&](uint32_t i, double dxd, double dyd, double d2d) {
+            const real4<R> vai = B.va[i];
+            const real4<R> pri = B.pred[i];
+            const R dx = (R)dxd, dy = (R)dyd;  // diff = r_i - r_j
+            const R d2 = (R)d2d;
+            const R dist = sqrt(d2);
+            const int ti = (int)vai.w;
+            // continuity density with v and rho of the current step
+            if (cont) {
+                const R F = kernel_gradf<R>(K, P.skf_density, dist);
+                const R gx = dx * F, gy = dy * F;
+                drho += m * ((vai.x - vaj.x) * gx + (vai.y - vaj.y) * gy);
+                if (delta) {
+                    const R rf = -(dx * gx + dy * gy) / d2;
+                    const R pf = (vai.z - rhoj) / vai.z;
+                    drho += dhc2m * (pf * rf);
+                }
+            }
+            // Lennard-Jones wall force on fluid from boundary particles
+            if (P.lj_on && tj > 0 && ti < 0 && dist < lj_r0) {
+                const R rn = lj_r0 / dist;
+                const R rn2 = rn * rn, rn4 = rn2 * rn2;
+                const R f = lj_D * (rn4 * rn4 * rn4 - rn4) / d2;
+                ax -= dx * f;
+                ay -= dy * f;
+            }
+            // pressure (predicted density, pressure of the current density)
+            {
+                const R F = kernel_gradf<R>(K, P.skf_intf, dist);
+                const R rhohi = pri.z, pi = pri.w;
+                R pf = P.intf_approx == 1 ? (pjres + pi) / (rhohj * rhohi) : pi / (rhohi * rhohi) + pj_rho2;
+                if (P.art_pressure && (pi < R(0) || pjres < R(0))) {
+                    const R fij = kernel_w<R>(K, P.skf_ap, dist) / (R)P.ap_wdelta;
+                    const R api = pi < R(0) ? (-(R)P.ap_coef * pi / (rhohi * rhohi)) : R(0);
+                    const R apj = pjres < R(0) ? (-(R)P.ap_coef * pjres / (rhohj * rhohj)) : R(0);
+                    pf += (api + apj) * pow(fij, (R)P.ap_index);
+                }
+                const R c = F * pf * m;
+                ax += dx * c;
+                ay += dy * c;
+            }
+            // artificial viscosity with the predicted velocity and density
+            if (P.av_on) {
+                const R dvx = prj.x - pri.x, dvy = prj.y - pri.y;
+                const R vr = -(dvx * dx + dvy * dy);  // (v_j - v_i) . (r_j - r_i)
+                if (vr < R(0)) {
+                    const R mu = h * vr / (d2 + eta2);
+                    amu = fmax(amu, fabs(mu));
+                    const R rhoij = R(0.5) * (pri.z + rhohj);
+                    const R piv = (beta * mu - alpha * cs) * mu / rhoij;
+                    const R F = kernel_gradf<R>(K, P.skf_av, dist);
+                    const R c = F * piv * m;
+                    ax += dx * c;
+                    ay += dy * c;
+                }
+            }
+            // XSPH with fluid neighbours only
+            if (P.xsph_on && ti > 0) {
+                const R w = kernel_w<R>(K, P.skf_xsph, dist);
+                const R c = m / (pri.z + rhohj) * w * two_eps;
+                avx += (pri.x - prj.x) * c;
+                avy += (pri.y - prj.y) * c;
+            }
+        });
+    }
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
+        wave_max_bits<R>(&ctrl->amax_bits, live ? (R)sqrt((double)ax * ax + (double)ay * ay) : R(0));
+        wave_max_bits<R>(&ctrl->mumax_bits, live ? amu : R(0));
+    }
+    if (EXPORT || !FUSED) {
+        if (live) {
+            real4<R> d;
+            d.x = ax;
+            d.y = ay;
+            d.z = drho;
+            d.w = amu;
+            der[s] = d;
+            real4<R> x;
+            x.x = avx;
+            x.y = avy;
+            x.z = R(0);
+            x.w = R(0);
+            xs[s] = x;
+        }
+    }
+    if (FUSED) {
+        uint32_t outside = 0, bad = 0;
+        if (lb == 0 && threadIdx.x == 0) {  // nobody reads these during the pass
+            ctrl->dt_new = ctrl->cur_dt_const;
+            ctrl->pending = 1;
+        }
+        if (live) {
+            const Ctrl& c = *ctrl;
+            integrate_one<R>(P, c, s, pj, vaj, prj, oid, ax, ay, drho, avx, avy, c.cur_dt_const, A, key_next,
+                             outside, bad);
+        }
+        wave_count_add(&ctrl->outside, outside);
+        if (bad) atomicOr(&ctrl->grid_overflow, 1u);
+    }
+}
+
+// ------------------------------------------------------------------ integrate (dynamic dt)
+// dt = CFL min(sqrt(h / max|a|), h / (c + max mu)) (UpdateAcceleration.cpp:15-29)
+template <class R>
+__global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
+                                                           const real4<R>* __restrict__ der,
+                                                           const real4<R>* __restrict__ xs, StateA<R> A,
+                                                           uint32_t* __restrict__ key_next) {
+    Ctrl& c = *ctrl;
+    if (c.halted) return;
+    double dt;
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
+        const double amax = dbits(c.amax_bits), mumax = dbits(c.mumax_bits);
+        dt = P.cfl * fmin(sqrt(P.h / amax), P.h / (P.c0 + mumax));
+    } else {
+        dt = c.cur_dt_const;
+    }
+    const bool never = P.dt_method == 2 && dt <= P.never_ending;
+    const bool nbr_err = P.consistency_check && c.nbr_overflow;
+    if (never || nbr_err) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c.halted = 1;
+            c.status = nbr_err ? 3u : 4u;
+        }
+        return;
+    }
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t outside = 0, bad = 0;
+    if (s < P.n) {
+        const real4<R> d = der[s];
+        const real4<R> x = xs[s];
+        integrate_one<R>(P, c, s, B.pos[s], B.va[s], B.pred[s], B.oid[s], d.x, d.y, d.z, x.x, x.y, dt, A, key_next,
+                         outside, bad);
+    }
+    wave_count_add(&c.outside, outside);
+    if (bad) atomicOr(&c.grid_overflow, 1u);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.dt_new = dt;
+        c.pending = 1;
+    }
+}
+
+// ------------------------------------------------------------------ utilities
+template <class R>
+__global__ void init_kernel(StepParams P, const double* __restrict__ r, const double* __restrict__ v,
+                            const double* __restrict__ rho, const int32_t* __restrict__ itype, StateA<R> A,
+                            uint32_t* __restrict__ key_next, Ctrl* __restrict__ ctrl) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    double2 p = make_double2(r[2 * s], r[2 * s + 1]);
+    A.pos[s] = p;
+    real4<R> va;
+    va.x = (R)v[2 * s];
+    va.y = (R)v[2 * s + 1];
+    va.z = (R)rho[s];
+    va.w = (R)itype[s];
+    A.va[s] = va;
+    real4<R> z{};
+    A.ad[s] = z;
+    A.oid[s] = s;
+    uint32_t bad = 0;
+    key_next[s] = itype[s] == 0 ? P.ncells : cell_key(P, p.x, p.y, bad);
+    if (bad) atomicOr(&ctrl->grid_overflow, 1u);
+}
+
+// Un-permute the state into original order (AoS doubles).
+template <class R>
+__global__ void download_kernel(StepParams P, StateA<R> A, const real4<R>* __restrict__ pred, double* __restrict__ r,
+                                double* __restrict__ v, double* __restrict__ rho, double* __restrict__ p,
+                                int32_t* __restrict__ itype) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const uint32_t o = A.oid[s];
+    const double2 pos = A.pos[s];
+    const real4<R> va = A.va[s];
+    r[2 * o] = pos.x;
+    r[2 * o + 1] = pos.y;
+    v[2 * o] = (double)va.x;
+    v[2 * o + 1] = (double)va.y;
+    rho[o] = (double)va.z;
+    itype[o] = (int32_t)va.w;
+    p[o] = pred ? (double)pred[s].w : (double)eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
+}
+
+template <class R>
+__global__ void fields_kernel(uint32_t n, const uint32_t* __restrict__ oid, const real4<R>* __restrict__ der,
+                              const real4<R>* __restrict__ xs, double* __restrict__ a, double* __restrict__ drho,
+                              double* __restrict__ av, double* __restrict__ mu) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t o = oid[s];
+    const real4<R> d = der[s];
+    const real4<R> x = xs[s];
+    a[2 * o] = (double)d.x;
+    a[2 * o + 1] = (double)d.y;
+    drho[o] = (double)d.z;
+    mu[o] = (double)d.w;
+    av[2 * o] = (double)x.x;
+    av[2 * o + 1] = (double)x.y;
+}
+
+// Morton cell key of the reference (GridUtils.h:46-151) per particle.
+__global__ void morton_kernel(StepParams P, const double2* __restrict__ pos, const uint32_t* __restrict__ oid,
+                              uint32_t* __restrict__ out) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const double2 p = pos[s];
+    const unsigned long long cx = (unsigned long long)((p.x - P.xmin) / P.cell);
+    const unsigned long long cy = (unsigned long long)((p.y - P.ymin) / P.cell);
+    out[oid[s]] = split2((uint32_t)cx) | (split2((uint32_t)cy) << 1);
+}
+
+// Neighbour lists in the reference layout (original indices, sentinel n).
+template <class R>
+__global__ void neighbours_kernel(StepParams P, StateB<R> B, const uint32_t* __restrict__ skey,
+                                  const uint32_t* __restrict__ LB, uint32_t* __restrict__ out, uint32_t max_nbr,
+                                  Ctrl* __restrict__ ctrl) {
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.n) return;
+    const uint32_t o = B.oid[s];
+    uint32_t cnt = 0;
+    bool overflow = false;
+    if ((int)B.va[s].w != 0) {
+        for_each_neighbour(P, max_nbr - 1, s, skey[s], B.pos[s], B.pos, LB, overflow,
+                           [&](uint32_t i, double, double, double) {
+            out[cnt + (size_t)max_nbr * o] = B.oid[i];
+            ++cnt;
+        });
+    }
+    out[cnt + (size_t)max_nbr * o] = P.n;
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+}
+
+}  // namespace rrsph
