@@ -89,8 +89,11 @@ def dam_break(delta: float = 0.01, tank=(3.2, 1.6), column=(1.0, 0.5), layers: i
     W, H = tank
     cw, ch = column
     nx, ny = int(round(cw / delta)), int(round(ch / delta))
-    xs = delta * (np.arange(nx) + 1.0)
-    ys = delta * (np.arange(ny) + 1.0)
+    # fluid starts at the Lennard-Jones zero-force distance r0 = 2h from the
+    # wall rows (ExtForce.h:26: the wall force acts below r0)
+    gap = 2.0 * hsml_coef * delta
+    xs = gap + delta * np.arange(nx)
+    ys = gap + delta * np.arange(ny)
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     fluid = np.stack([X.ravel(), Y.ravel()], axis=1)
     rng = np.random.default_rng(seed)
@@ -131,18 +134,20 @@ def piston_tank(delta: float = 0.0075, length: float = 20.0, depth: float = 0.5,
     a vertical piston wall on the left moving as x(t) = A sin(2 pi t / T) (or a
     Fourier series), 3-layer LJ walls, continuity density, dynamic CFL dt."""
     nx, ny = int(round(length / delta)), int(round(depth / delta))
-    xs = delta * (np.arange(nx) + 1.0)
-    ys = delta * (np.arange(ny) + 0.5)
+    gap = 2.0 * hsml_coef * delta  # Lennard-Jones zero-force distance r0 = 2h
+    xs = gap + delta * np.arange(nx)
+    ys = gap + delta * np.arange(ny)
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     fluid = np.stack([X.ravel(), Y.ravel()], axis=1)
     if jitter:
         rng = np.random.default_rng(seed)
         fluid = fluid + rng.uniform(-jitter, jitter, size=fluid.shape)
-    L_end = length + delta
-    piston = _wall_line((0.0, 0.5 * delta), (0, 1), (-1, 0), wall_height, delta, layers)
-    bottom = _wall_line((-(layers + 1) * delta, -0.5 * delta), (1, 0), (0, -1),
+    L_end = xs[-1] + gap
+    surface = ys[-1] + 0.5 * delta
+    piston = _wall_line((0.0, delta), (0, 1), (-1, 0), wall_height, delta, layers)
+    bottom = _wall_line((-(layers + 1) * delta, 0.0), (1, 0), (0, -1),
                         L_end + 2 * (layers + 1) * delta, delta, layers)
-    right = _wall_line((L_end, 0.5 * delta), (0, 1), (1, 0), wall_height, delta, layers)
+    right = _wall_line((L_end, delta), (0, 1), (1, 0), wall_height, delta, layers)
     nfluid = len(fluid)
     r = np.concatenate([fluid, piston, bottom, right])
     n = len(r)
@@ -170,8 +175,7 @@ def piston_tank(delta: float = 0.0075, length: float = 20.0, depth: float = 0.5,
     P.fill_in_computing_params(prm)
     P.post_fill_in_model_params(prm)
     v = np.zeros_like(r)
-    rho = hydrostatic_rho(r[:, 1], depth, prm.rho0, prm.eos_c)
-    rho[nfluid:] = hydrostatic_rho(np.maximum(r[nfluid:, 1], -1.0), depth, prm.rho0, prm.eos_c)
+    rho = hydrostatic_rho(r[:, 1], surface, prm.rho0, prm.eos_c)
     itype = np.concatenate([np.full(nfluid, P.TYPE_WATER), np.full(n - nfluid, P.TYPE_BOUNDARY)]).astype(np.int32)
     return Layout(prm, r, v, rho, itype)
 
