@@ -1,0 +1,195 @@
+"""Benchmark of the RRSPH_HIP WCSPH step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5]
+
+Workload (default): the 2D piston wave flume at ~1M particles ("C1-1M"):
+20 m x 0.5 m of water at spacing 0.0031623 m (999,350 fluid particles,
+1,020,266 in total with 3-layer Lennard-Jones walls), continuity density,
+Tait EOS, artificial viscosity, XSPH, piston x(t) = 0.05 sin(2 pi t / 1.2),
+dynamic CFL dt from the max-|a| / max-mu reductions.  A "step" is one whole
+time step (sort, predict, density, forces, dt, integrate) over all particles.
+
+Prints ONE JSON line.  `value` is particle-steps/s of the FP32 path (FP32
+pair arithmetic with FP64 positions, dt and reductions) with the state
+resident in HBM; the FP64 path is timed beside it.  `roofline` is the
+dominant kernel's algorithmic bytes per launch over its average duration
+(HIP events on the context's stream); `cpu_baseline` is the FP64 CPU oracle
+(a port of the reference's RRSPH_OMP step) on a bounded sample of the same
+workload, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+# Algorithmic bytes per particle for the pair pass (continuity, XSPH, AV,
+# dynamic dt) and for the whole step: the canonical pipeline's compulsory
+# traffic, one read of every input and one write of every output per kernel.
+B_ALG_PAIR = {"fp32": 64, "fp64": 108}
+B_ALG_STEP = {"fp32": 304, "fp64": 440}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--delta", type=float, default=None, help="particle spacing (default: the ~1M flume)")
+    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps in the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=10)
+    return ap.parse_args()
+
+
+def gpu_leg(layout, precision, steps, warmup, profile_steps):
+    import torch
+
+    from sph2d_amd import hip as H
+
+    prm = layout.params
+    g = H.HipStep(prm, precision=precision)
+    g.upload(layout.r, layout.v, layout.rho, layout.itype)
+    if warmup:
+        g.step(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    wall_s = t1 - t0
+    # the same number of steps between two HIP events on the context's stream
+    ev_ms = g.time_steps(steps)
+    # per-kernel averages: HIP events around every launch on the context's stream
+    per_kernel, prof_total = g.profile_steps(profile_steps)
+    t, dt, step = g.time()
+    diag = g.diagnostics()
+    g.close()
+    return {
+        "wall_s": wall_s,
+        "event_ms": ev_ms,
+        "per_kernel_ms": per_kernel,
+        "profiled_step_ms": prof_total / max(profile_steps, 1),
+        "time": t,
+        "dt": dt,
+        "step": step,
+        "diag": diag,
+    }
+
+
+def cpu_leg(layout, steps):
+    import oracle
+
+    prm = layout.params
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    o = oracle.Oracle(prm, num_threads=threads)
+    o.set_state(layout.r, layout.v, layout.rho, layout.itype, time=0.0, dt=prm.dt)
+    o.step(2)
+    t0 = time.perf_counter()
+    st = o.step(steps)
+    t1 = time.perf_counter()
+    if st != 0:
+        raise RuntimeError(f"oracle status {st}")
+    return prm.ntotal * steps / (t1 - t0), threads, t1 - t0
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from sph2d_amd import generators as G
+    from sph2d_amd import hip as H
+
+    torch.cuda.set_device(0)
+    delta = args.delta or G.c1_1m_delta()
+    layout = G.piston_tank(delta=delta)
+    prm = layout.params
+    n = prm.ntotal
+
+    r32 = gpu_leg(layout, H.FP32, args.steps, args.warmup, args.profile_steps)
+    value = n * args.steps / r32["wall_s"]
+    r64 = None
+    if not args.no_fp64:
+        r64 = gpu_leg(layout, H.FP64, args.steps, args.warmup, args.profile_steps)
+
+    # dominant kernel of the FP32 step
+    pk = r32["per_kernel_ms"]
+    dom = max((k for k in pk if pk[k] > 0), key=lambda k: pk[k] * (3 if "radix" in k else 1))
+    pair_ms = pk["pair_kernel"]
+    achieved = B_ALG_PAIR["fp32"] * n / (pair_ms * 1e-3) / 1e9
+    roofline = {
+        "kernel": "pair_kernel",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_particle": B_ALG_PAIR["fp32"],
+        "avg_launch_ms": round(pair_ms, 5),
+        "step_frac": round(value * B_ALG_STEP["fp32"] / 1e9 / HBM_PEAK_GBS, 4),
+        "dominant_by_time": dom,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "pair_kernel_c1_1m_fp32_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu_rate, cores, secs = cpu_leg(layout, args.cpu_steps)
+        cpu = {
+            "value": round(cpu_rate, 1),
+            "unit": "particle-steps/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {n}-particle flume after 2 warm-up steps "
+                      f"({secs:.1f} s), FP64 OpenMP oracle",
+        }
+
+    out = {
+        "metric": "particle-steps/s at 1M 2D (FP32+comp vs FP64); adjoint/forward cost ratio",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["wall_s"] / args.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "C1-1M: 2D piston wave flume 20 m x 0.5 m, delta=%.7g m, %d particles (%d fluid), "
+                        "continuity density, dynamic CFL dt, FP32 pair arithmetic with FP64 positions"
+                        % (delta, n, prm.nfluid),
+            "particles": n,
+            "precision": "fp32 pairs, fp64 positions/dt/reductions",
+        },
+        "event_ms_per_step": round(r32["event_ms"] / args.steps, 5),
+        "per_kernel_ms": {k: round(v, 5) for k, v in pk.items()},
+        "fp64": None if r64 is None else {
+            "value": round(n * args.steps / r64["wall_s"], 1),
+            "ms_per_step": round(r64["wall_s"] / args.steps * 1e3, 5),
+            "per_kernel_ms": {k: round(v, 5) for k, v in r64["per_kernel_ms"].items()},
+        },
+        "fp32_over_fp64": None if r64 is None else round(r64["wall_s"] / r32["wall_s"], 3),
+        "adjoint_over_forward": None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -3,8 +3,8 @@
 // (RRSPH_CL/cl/GridFind.cl:73-93, CLTimeIntegration.cpp:170-181) and the
 // serial counting sort of RRSPH_OMP (GridFind.cpp:47-88).
 //
-// One pass = upsweep (per-tile digit histograms) -> one-block scan over the
-// digit-major histogram table -> downsweep (stable in-tile ranking with wave
+// One pass = upsweep (per-tile digit histograms) -> per-digit scan across
+// tiles (one block per digit) -> downsweep (stable in-tile ranking with wave
 // ballots, scatter).  Tiles are kSortTile keys; within a tile the order of
 // equal digits is preserved, and tiles are placed in block order, so the
 // sort is stable and deterministic.
@@ -54,32 +54,41 @@ __global__ void __launch_bounds__(kBlock) radix_upsweep(const uint32_t* __restri
     hist[tid * ntiles + blockIdx.x] = s_hist[tid];
 }
 
-// Exclusive scan of `total` entries in place, one block of 1024 threads.
-__global__ void __launch_bounds__(1024) radix_scan(uint32_t* __restrict__ hist, uint32_t total,
-                                                   const Ctrl* __restrict__ ctrl) {
+// Per-digit exclusive scan across tiles: block d scans hist[d * ntiles + t]
+// over t in place and stores the digit's total in digit_total[d].
+constexpr int kScanBlock = 512;
+__global__ void __launch_bounds__(kScanBlock) radix_scan_digits(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                                uint32_t* __restrict__ digit_total,
+                                                                const Ctrl* __restrict__ ctrl) {
     if (ctrl->halted) return;
-    __shared__ uint32_t s_part[1024];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t chunk = (total + 1023) / 1024;
-    const uint32_t b0 = tid * chunk;
-    const uint32_t b1 = min(b0 + chunk, total);
-    uint32_t sum = 0;
-    for (uint32_t k = b0; k < b1; ++k) sum += hist[k];
-    s_part[tid] = sum;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = tid >= off ? s_part[tid - off] : 0;
+    __shared__ uint32_t s_w[kScanBlock / kWave];
+    const uint32_t tid = threadIdx.x, lane = __lane_id(), wave = tid / kWave;
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < ntiles; base += kScanBlock) {
+        const uint32_t t = base + tid;
+        const uint32_t v = t < ntiles ? row[t] : 0;
+        // inclusive wave scan
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)lane >= off) x += y;
+        }
+        if (lane == kWave - 1) s_w[wave] = x;
         __syncthreads();
-        s_part[tid] += v;
+        uint32_t wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kScanBlock / kWave; ++w) {
+            const uint32_t sw = s_w[w];
+            if (w < (int)wave) wbase += sw;
+            total += sw;
+        }
+        if (t < ntiles) row[t] = carry + wbase + x - v;
+        carry += total;
         __syncthreads();
     }
-    uint32_t run = tid ? s_part[tid - 1] : 0;
-    for (uint32_t k = b0; k < b1; ++k) {
-        const uint32_t v = hist[k];
-        hist[k] = run;
-        run += v;
-    }
+    if (tid == 0) digit_total[blockIdx.x] = carry;
 }
 
 // Stable scatter of one pass.  vals_in == nullptr means vals = index.
@@ -88,15 +97,32 @@ __global__ void __launch_bounds__(kBlock) radix_downsweep(const uint32_t* __rest
                                                           uint32_t* __restrict__ keys_out,
                                                           uint32_t* __restrict__ vals_out, uint32_t n, uint32_t shift,
                                                           const uint32_t* __restrict__ hist_scanned, uint32_t ntiles,
+                                                          const uint32_t* __restrict__ digit_total,
                                                           const Ctrl* __restrict__ ctrl) {
     if (ctrl->halted) return;
     constexpr int kWaves = kBlock / kWave;
+    __shared__ uint32_t s_tot[kWaves];
     __shared__ uint32_t s_base[kRadix];          // global start of digit d for this tile + digits seen so far
     __shared__ uint32_t s_wcnt[kWaves][kRadix];  // per-wave digit counts of the current iteration
     __shared__ uint32_t s_wofs[kWaves][kRadix];  // per-wave digit offsets of the current iteration
     const uint32_t tid = threadIdx.x;
     const uint32_t wave = tid / kWave;
-    s_base[tid] = hist_scanned[tid * ntiles + blockIdx.x];
+    {
+        // exclusive scan of the 256 digit totals (one digit per thread)
+        const uint32_t lane = __lane_id();
+        const uint32_t v = digit_total[tid];
+        uint32_t x = v;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if ((int)lane >= off) x += y;
+        }
+        if (lane == kWave - 1) s_tot[wave] = x;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (uint32_t w = 0; w < wave; ++w) wbase += s_tot[w];
+        s_base[tid] = wbase + x - v + hist_scanned[tid * ntiles + blockIdx.x];
+    }
     const uint32_t base = blockIdx.x * kSortTile;
     for (int it = 0; it < kSortItems; ++it) {
 #pragma unroll

@@ -87,10 +87,13 @@ struct rrsph_ctx {
     // buffers
     double2 *posA = nullptr, *posB = nullptr;
     void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
-         *pred_scratch = nullptr;
+         *pred_scratch = nullptr, *pres = nullptr, *pres_scratch = nullptr;
+    bool fast_pair = false;
     uint32_t *oidA = nullptr, *oidB = nullptr;
     uint32_t *key_next = nullptr, *kbuf[2] = {nullptr, nullptr}, *vbuf[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;
+    uint32_t* digit_total = nullptr;
+    unsigned long long* maxslots = nullptr;
     uint32_t* LB = nullptr;
     Ctrl* ctrl = nullptr;
     Ctrl* util_ctrl = nullptr;
@@ -124,8 +127,8 @@ rrsph_status fail(rrsph_ctx* c, rrsph_status st, const std::string& msg) {
 template <class R> StateA<R> stateA(rrsph_ctx* c) {
     return StateA<R>{c->posA, (real4<R>*)c->vaA, (real4<R>*)c->adA, c->oidA};
 }
-template <class R> StateB<R> stateB(rrsph_ctx* c, void* pred) {
-    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB};
+template <class R> StateB<R> stateB(rrsph_ctx* c, void* pred, void* pres) {
+    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB, (R*)pres};
 }
 
 uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
@@ -160,9 +163,10 @@ void enqueue_sort(rrsph_ctx* c, Ctrl* ctrl, Launcher& L) {
         const uint32_t shift = p * kRadixBits;
         const uint32_t nt = c->ntiles;
         L(K_UPSWEEP, [&] { radix_upsweep<<<nt, kBlock, 0, c->stream>>>(src_k, n, shift, c->hist, nt, ctrl); });
-        L(K_SCAN, [&] { radix_scan<<<1, 1024, 0, c->stream>>>(c->hist, kRadix * nt, ctrl); });
+        L(K_SCAN, [&] { radix_scan_digits<<<kRadix, kScanBlock, 0, c->stream>>>(c->hist, nt, c->digit_total, ctrl); });
         L(K_DOWNSWEEP, [&] {
-            radix_downsweep<<<nt, kBlock, 0, c->stream>>>(src_k, src_v, dk, dv, n, shift, c->hist, nt, ctrl);
+            radix_downsweep<<<nt, kBlock, 0, c->stream>>>(src_k, src_v, dk, dv, n, shift, c->hist, nt, c->digit_total,
+                                                          ctrl);
         });
         src_k = dk;
         src_v = dv;
@@ -179,9 +183,10 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
     L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
     enqueue_sort(c, c->ctrl, L);
     StateA<R> A = stateA<R>(c);
-    StateB<R> B = stateB<R>(c, c->pred);
+    StateB<R> B = stateB<R>(c, c->pred, c->pres);
     L(K_REORDER, [&] {
-        reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB);
+        reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
+                                                        c->maxslots);
     });
     if (P.density_treatment == 0) {
         L(K_DENSITY_SUM, [&] {
@@ -190,25 +195,25 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
     }
     real4<R>* der = (real4<R>*)c->der;
     real4<R>* xs = (real4<R>*)c->xs;
+    auto pair = [&](auto kernel) {
+        L(K_PAIR, [&] {
+            kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A, c->key_next, der, xs,
+                                                     c->maxslots);
+        });
+    };
     if (c->fused) {
         if (c->export_fields) {
-            L(K_PAIR, [&] {
-                pair_kernel<R, true, true><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A,
-                                                                            c->key_next, der, xs);
-            });
+            if (c->fast_pair) pair(pair_fast_kernel<R, true, true>);
+            else pair(pair_kernel<R, true, true>);
         } else {
-            L(K_PAIR, [&] {
-                pair_kernel<R, true, false><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB,
-                                                                             A, c->key_next, der, xs);
-            });
+            if (c->fast_pair) pair(pair_fast_kernel<R, true, false>);
+            else pair(pair_kernel<R, true, false>);
         }
     } else {
-        L(K_PAIR, [&] {
-            pair_kernel<R, false, true><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A,
-                                                                         c->key_next, der, xs);
-        });
+        if (c->fast_pair) pair(pair_fast_kernel<R, false, true>);
+        else pair(pair_kernel<R, false, true>);
         L(K_INTEGRATE, [&] {
-            integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next);
+            integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots);
         });
     }
 }
@@ -241,8 +246,9 @@ template <class R> void enqueue_util_sort(rrsph_ctx* c) {
     enqueue_sort(c, c->util_ctrl, L);
     const uint32_t nb = grid_blocks(c->P.n, kBlock);
     StateA<R> A = stateA<R>(c);
-    StateB<R> B = stateB<R>(c, c->pred_scratch);
-    reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB);
+    StateB<R> B = stateB<R>(c, c->pred_scratch, c->pres_scratch);
+    reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
+                                                    c->maxslots + 2 * kMaxSlots);
 }
 
 std::string status_message(uint32_t st) {
@@ -384,13 +390,18 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     ctx->ntiles = grid_blocks(P.n, kSortTile);
     ctx->fp32 = p.precision == RRSPH_FP32;
     ctx->fused = !(p.dt_correction_method == 2 && p.dt_dynamic_mode == 1);
+    ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&
+                     p.average_velocity_skf == 1 && p.intf_sph_approximation == 2 && !p.artificial_pressure &&
+                     (p.density_treatment == 0 || p.density_treatment == 1) && std::fabs(p.cell_scale_k - 2.0) == 0.0;
     return RRSPH_OK;
 }
 
 void free_all(rrsph_ctx* c) {
     drop_graph(c);
-    void* ptrs[] = {c->posA, c->posB, c->vaA, c->vaB, c->adA, c->pred, c->der, c->xs, c->pred_scratch, c->oidA,
-                    c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->LB, c->ctrl,
+    void* ptrs[] = {c->posA, c->posB, c->vaA, c->vaB, c->adA, c->pred, c->der, c->xs, c->pred_scratch, c->pres,
+                    c->pres_scratch, c->oidA,
+                    c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->digit_total,
+                    c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32};
     for (void* p : ptrs)
@@ -434,6 +445,8 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->der = dalloc<char>(n * r4, e);
     ctx->xs = dalloc<char>(n * r4, e);
     ctx->pred_scratch = dalloc<char>(n * r4, e);
+    ctx->pres = dalloc<char>(n * (r4 / 4), e);
+    ctx->pres_scratch = dalloc<char>(n * (r4 / 4), e);
     ctx->oidA = dalloc<uint32_t>(n, e);
     ctx->oidB = dalloc<uint32_t>(n, e);
     ctx->key_next = dalloc<uint32_t>(n, e);
@@ -442,6 +455,8 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->vbuf[0] = dalloc<uint32_t>(n, e);
     ctx->vbuf[1] = dalloc<uint32_t>(n, e);
     ctx->hist = dalloc<uint32_t>((size_t)kRadix * ctx->ntiles, e);
+    ctx->digit_total = dalloc<uint32_t>(kRadix, e);
+    ctx->maxslots = dalloc<unsigned long long>(4 * kMaxSlots, e);  // second half: utility sorts
     ctx->LB = dalloc<uint32_t>((size_t)ctx->P.ncells + 2, e);
     ctx->ctrl = dalloc<Ctrl>(1, e);
     ctx->util_ctrl = dalloc<Ctrl>(1, e);
@@ -582,11 +597,11 @@ rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, doub
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (ctx->fp32)
         download_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(
-            ctx->P, stateA<float>(ctx), ctx->stepped ? (const float4*)ctx->pred : nullptr, ctx->st_r, ctx->st_v,
+            ctx->P, stateA<float>(ctx), ctx->stepped ? (const float*)ctx->pres : nullptr, ctx->st_r, ctx->st_v,
             ctx->st_rho, ctx->st_p, ctx->st_itype);
     else
         download_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(
-            ctx->P, stateA<double>(ctx), ctx->stepped ? (const double4*)ctx->pred : nullptr, ctx->st_r, ctx->st_v,
+            ctx->P, stateA<double>(ctx), ctx->stepped ? (const double*)ctx->pres : nullptr, ctx->st_r, ctx->st_v,
             ctx->st_rho, ctx->st_p, ctx->st_itype);
     HIP_TRY(hipGetLastError());
     if ((fields & RRSPH_FIELD_R) && r)
@@ -656,12 +671,12 @@ rrsph_status rrsph_hip_neighbours(rrsph_ctx* ctx, uint32_t* out, uint32_t max_nb
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (ctx->fp32) {
         enqueue_util_sort<float>(ctx);
-        neighbours_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<float>(ctx, ctx->pred_scratch),
+        neighbours_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<float>(ctx, ctx->pred_scratch, ctx->pres_scratch),
                                                                  ctx->sorted_keys, ctx->LB, dout, max_nbr,
                                                                  ctx->util_ctrl);
     } else {
         enqueue_util_sort<double>(ctx);
-        neighbours_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<double>(ctx, ctx->pred_scratch),
+        neighbours_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch),
                                                                   ctx->sorted_keys, ctx->LB, dout, max_nbr,
                                                                   ctx->util_ctrl);
     }

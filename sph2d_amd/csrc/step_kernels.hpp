@@ -5,7 +5,9 @@
 //   va    real4     (vx, vy, rho, itype)
 //   ad    real4     (ax, ay, drho, 0) of the previous step (predictor input)
 //   oid   uint32    original particle index
-//   pred  real4     (vx^, vy^, rho^, p): predicted velocity and density, pressure
+//   pred  real4     (vx^, vy^, rho^, p / rho^2): predicted velocity and density,
+//                   pressure over predicted density squared
+//   pres  real      p of the current density (the reference's p array)
 // Two copies of (pos, va, oid) ping-pong: "A" holds the state between steps,
 // "B" the freshly sorted copy a step reads from.
 //
@@ -29,8 +31,9 @@ template <class R> struct StateA {
 template <class R> struct StateB {
     double2* pos;
     real4<R>* va;
-    real4<R>* pred;
+    real4<R>* pred;   // (vx^, vy^, rho^, p / rho^2)
     uint32_t* oid;
+    R* pres;          // p, the reference's pressure array
 };
 
 // ------------------------------------------------------------------ grid keys
@@ -85,6 +88,56 @@ __device__ double piston_velocity(const StepParams& P, double t) {
 
 __device__ __forceinline__ double dbits(unsigned long long b) { return __longlong_as_double((long long)b); }
 
+// Block maximum of two non-negative values, folded into one of kMaxSlots
+// slots per quantity with an atomic max on the bit pattern (spreading the
+// atomics over many addresses keeps them from serialising on one).
+constexpr uint32_t kMaxSlots = 256;
+__device__ __forceinline__ void block_max2_to_slots(double a, double b, unsigned long long* __restrict__ slots,
+                                                    uint32_t blk) {
+    __shared__ double s_a[kBlock / kWave], s_b[kBlock / kWave];
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fmax(a, __shfl_xor(a, off));
+        b = fmax(b, __shfl_xor(b, off));
+    }
+    const uint32_t w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        s_a[w] = a;
+        s_b[w] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBlock / kWave; ++k) {
+            a = fmax(a, s_a[k]);
+            b = fmax(b, s_b[k]);
+        }
+        const uint32_t slot = blk % kMaxSlots;
+        if (a > 0.0) atomicMax(&slots[slot], (unsigned long long)__double_as_longlong(a));
+        if (b > 0.0) atomicMax(&slots[kMaxSlots + slot], (unsigned long long)__double_as_longlong(b));
+    }
+}
+
+// Maximum over the slots, computed by every block (deterministic).
+__device__ __forceinline__ void slots_max2(const unsigned long long* __restrict__ slots, double& a, double& b) {
+    __shared__ double s_a[kBlock / kWave], s_b[kBlock / kWave];
+    double x = dbits(slots[threadIdx.x]), y = dbits(slots[kMaxSlots + threadIdx.x]);
+    for (int off = 32; off > 0; off >>= 1) {
+        x = fmax(x, __shfl_xor(x, off));
+        y = fmax(y, __shfl_xor(y, off));
+    }
+    const uint32_t w = threadIdx.x / kWave;
+    if (__lane_id() == 0) {
+        s_a[w] = x;
+        s_b[w] = y;
+    }
+    __syncthreads();
+    a = s_a[0];
+    b = s_b[0];
+    for (int k = 1; k < kBlock / kWave; ++k) {
+        a = fmax(a, s_a[k]);
+        b = fmax(b, s_b[k]);
+    }
+}
+
 // One thread: commit the previous step, decide whether to halt, snapshot.
 __global__ void step_prologue(StepParams P, Ctrl* ctrl) {
     Ctrl& c = *ctrl;
@@ -138,32 +191,62 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
                                                          const uint32_t* __restrict__ skey,
                                                          const uint32_t* __restrict__ perm, StateA<R> A, StateB<R> B,
-                                                         uint32_t* __restrict__ LB) {
+                                                         uint32_t* __restrict__ LB,
+                                                         unsigned long long* __restrict__ maxslots) {
     if (ctrl->halted) return;
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.n) return;
+    if (blockIdx.x == 0) {  // clear the max slots of this step's reductions
+        maxslots[threadIdx.x] = 0;
+        maxslots[kMaxSlots + threadIdx.x] = 0;
+    }
+    const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = s0 < P.n;
+    const uint32_t s = live ? s0 : P.n - 1;  // idle lanes shadow the last particle (no writes)
     const uint32_t k = skey[s];
     const uint32_t src = perm[s];
     const double2 pos = A.pos[src];
     const real4<R> va = A.va[src];
     const real4<R> ad = A.ad[src];
     const uint32_t oid = A.oid[src];
-    B.pos[s] = pos;
-    B.va[s] = va;
-    B.oid[s] = oid;
+    if (live) {
+        B.pos[s] = pos;
+        B.va[s] = va;
+        B.oid[s] = oid;
+    }
     const R dt = (R)ctrl->cur_dt;
     const bool cont = P.density_treatment != 0;
     real4<R> pr;
     pr.x = va.x + ad.x * dt * R(0.5);
     pr.y = va.y + ad.y * dt * R(0.5);
     pr.z = cont ? va.z + ad.z * dt * R(0.5) : va.z;
-    pr.w = eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
-    B.pred[s] = pr;
-    // cell-start table over keys 0 .. ncells (+1 sentinel entry)
-    const uint32_t prev = s == 0 ? 0u : skey[s - 1] + 1u;
-    for (uint32_t c = prev; c <= k; ++c) LB[c] = s;
-    if (s == P.n - 1)
-        for (uint32_t c = k + 1; c <= P.ncells + 1; ++c) LB[c] = P.n;
+    const R p = eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
+    pr.w = p / (pr.z * pr.z);
+    if (live) {
+        B.pred[s] = pr;
+        B.pres[s] = p;
+    }
+    // cell-start table over keys 0 .. ncells (+1 sentinel entry): lane s
+    // owns the cells (key[s-1], key[s]]; the last lane also owns the cells
+    // after its key.  Short runs are written by the owning lane, long runs
+    // (empty columns above the water, the domain margins) by the whole wave.
+    uint32_t lo = s == 0 ? 0u : skey[s - 1] + 1u;
+    uint32_t hi = live ? k + 1 : lo;  // exclusive
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t val = pass == 0 ? s : P.n;
+        if (pass == 1) {
+            lo = live && s == P.n - 1 ? k + 1 : 0u;
+            hi = live && s == P.n - 1 ? P.ncells + 2 : 0u;
+        }
+        const bool small = hi - lo <= 4;
+        if (small)
+            for (uint32_t c = lo; c < hi; ++c) LB[c] = val;
+        unsigned long long big = __ballot(!small);
+        while (big) {
+            const int src = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const uint32_t blo = __shfl(lo, src), bhi = __shfl(hi, src), bval = __shfl(val, src);
+            for (uint32_t c = blo + __lane_id(); c < bhi; c += kWave) LB[c] = bval;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ neighbour walk
@@ -171,7 +254,11 @@ __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctr
 // |r_i - r_s|^2 < (k h)^2 decided in FP64 without contraction, in the
 // reference's cell order; stops adding after max_nbr - 1 neighbours and
 // reports the overflow (GridFind.cpp:144-176).
-template <class F>
+// With kFilter (FP32 pair arithmetic) the distance test runs in FP32 on the
+// FP64 differences and falls back to the exact FP64 test only inside a
+// relative band of 1e-5 around the cut-off, so the neighbour sets are those
+// of the FP64 test.
+template <bool kFilter, class F>
 __device__ __forceinline__ void for_each_neighbour(const StepParams& P, uint32_t cap, uint32_t s, uint32_t key,
                                                    double2 pj, const double2* __restrict__ pos,
                                                    const uint32_t* __restrict__ LB, bool& overflow, F&& f) {
@@ -179,6 +266,7 @@ __device__ __forceinline__ void for_each_neighbour(const StepParams& P, uint32_t
     const int cy = (int)key - cx * P.ncy;
     const int y0 = cy > 0 ? cy - 1 : 0;
     const int y1 = cy + 1 < P.ncy ? cy + 1 : P.ncy - 1;
+    const float cut_lo = (float)(P.cut2 * (1.0 - 1e-5)), cut_hi = (float)(P.cut2 * (1.0 + 1e-5));
     uint32_t count = 0;
 #pragma unroll 1
     for (int dc = -1; dc <= 1; ++dc) {
@@ -186,20 +274,32 @@ __device__ __forceinline__ void for_each_neighbour(const StepParams& P, uint32_t
         if (col < 0 || col >= P.ncx) continue;
         const uint32_t b = LB[col * P.ncy + y0];
         const uint32_t e = LB[col * P.ncy + y1 + 1];
+        double2 nxt = b < e ? pos[b] : pj;
 #pragma unroll 1
         for (uint32_t i = b; i < e; ++i) {
+            const double2 pi = nxt;
+            if (i + 1 < e) nxt = pos[i + 1];
             if (i == s) continue;
-            const double2 pi = pos[i];
             const double dx = __dsub_rn(pi.x, pj.x);
             const double dy = __dsub_rn(pi.y, pj.y);
-            const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
-            if (!(d2 < P.cut2)) continue;
+            if (kFilter) {
+                const float fx = (float)dx, fy = (float)dy;
+                const float f2 = fx * fx + fy * fy;
+                if (f2 >= cut_hi) continue;
+                if (f2 > cut_lo) {
+                    const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+                    if (!(d2 < P.cut2)) continue;
+                }
+            } else {
+                const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+                if (!(d2 < P.cut2)) continue;
+            }
             if (count == cap) {
                 overflow = true;
                 continue;
             }
             ++count;
-            f(i, dx, dy, d2);
+            f(i, dx, dy);
         }
     }
 }
@@ -223,15 +323,18 @@ __global__ void __launch_bounds__(kBlock) density_sum_kernel(StepParams P, Ctrl*
     bool overflow = false;
     if ((int)va.w != 0) {
         const double2 pj = B.pos[s];
-        for_each_neighbour(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, [&](uint32_t, double, double, double d2) {
-            rho += m * kernel_w<R>(K, P.skf_density, (R)sqrt(d2));
-        });
+        for_each_neighbour<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
+                                  [&](uint32_t, double dx, double dy) {
+                                      rho += m * kernel_w<R>(K, P.skf_density, (R)sqrt(dx * dx + dy * dy));
+                                  });
     }
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     real4<R> pr = B.pred[s];
+    const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
     pr.z = rho;
-    pr.w = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
+    pr.w = p / (rho * rho);
     B.pred[s] = pr;
+    B.pres[s] = p;
 }
 
 // ------------------------------------------------------------------ whole_step
@@ -304,10 +407,51 @@ __device__ __forceinline__ void wave_count_add(uint32_t* dst, uint32_t v) {
     if (__lane_id() == 0) atomicAdd(dst, sum);
 }
 
-template <class R> __device__ __forceinline__ void wave_max_bits(unsigned long long* dst, R v) {
-    double m = (double)v;
-    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
-    if (__lane_id() == 0 && m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+
+// Shared tail of the pair passes: dt reductions, optional export of the
+// fields, and whole_step fused in when dt is known before the step.
+template <class R, bool FUSED, bool EXPORT>
+__device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restrict__ ctrl, uint32_t lb, uint32_t s,
+                                              bool live, bool overflow, double2 pj, const real4<R>& vaj,
+                                              const real4<R>& prj, uint32_t oid, R ax, R ay, R drho, R avx, R avy,
+                                              R amu, StateA<R> A, uint32_t* __restrict__ key_next,
+                                              real4<R>* __restrict__ der, real4<R>* __restrict__ xs,
+                                              unsigned long long* __restrict__ maxslots) {
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
+        block_max2_to_slots(live ? sqrt((double)ax * ax + (double)ay * ay) : 0.0, live ? (double)amu : 0.0,
+                            maxslots, lb);
+    }
+    if (EXPORT || !FUSED) {
+        if (live) {
+            real4<R> d;
+            d.x = ax;
+            d.y = ay;
+            d.z = drho;
+            d.w = amu;
+            der[s] = d;
+            real4<R> x;
+            x.x = avx;
+            x.y = avy;
+            x.z = R(0);
+            x.w = R(0);
+            xs[s] = x;
+        }
+    }
+    if (FUSED) {
+        uint32_t outside = 0, bad = 0;
+        if (lb == 0 && threadIdx.x == 0) {  // nobody reads these during the pass
+            ctrl->dt_new = ctrl->cur_dt_const;
+            ctrl->pending = 1;
+        }
+        if (live) {
+            const Ctrl& c = *ctrl;
+            integrate_one<R>(P, c, s, pj, vaj, prj, oid, ax, ay, drho, avx, avy, c.cur_dt_const, A, key_next,
+                             outside, bad);
+        }
+        wave_count_add(&ctrl->outside, outside);
+        if (bad) atomicOr(&ctrl->grid_overflow, 1u);
+    }
 }
 
 // ------------------------------------------------------------------ pair pass
@@ -321,7 +465,8 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
                                                       const uint32_t* __restrict__ skey,
                                                       const uint32_t* __restrict__ LB, StateA<R> A,
                                                       uint32_t* __restrict__ key_next, real4<R>* __restrict__ der,
-                                                      real4<R>* __restrict__ xs) {
+                                                      real4<R>* __restrict__ xs,
+                                                      unsigned long long* __restrict__ maxslots) {
     if (ctrl->halted) return;
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
@@ -345,14 +490,15 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
         const bool cont = P.density_treatment != 0;
         const bool delta = P.density_treatment == 2;
         const R h = K.h;
-        const R rhoj = vaj.z, rhohj = prj.z, pjres = prj.w;
+        const R rhoj = vaj.z, rhohj = prj.z, pjres = B.pres[s];
         const R pj_rho2 = pjres / (rhohj * rhohj);
         const R alpha = (R)P.av_alpha, beta = (R)P.av_beta, cs = (R)P.av_c, eta2 = (R)P.av_eta2;
         const R two_eps = R(2) * (R)P.xsph_eps;
         const R lj_r0 = (R)P.lj_r0, lj_D = (R)P.lj_D;
         const R dhc2m = R(2) * (R)P.dhc * m;
-        for_each_neighbour(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
-                           [&](uint32_t i, double dxd, double dyd, double d2d) {
+        for_each_neighbour<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
+                           [&](uint32_t i, double dxd, double dyd) {
+            const double d2d = dxd * dxd + dyd * dyd;
             const real4<R> vai = B.va[i];
             const real4<R> pri = B.pred[i];
             const R dx = (R)dxd, dy = (R)dyd;  // diff = r_i - r_j
@@ -381,7 +527,7 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
             // pressure (predicted density, pressure of the current density)
             {
                 const R F = kernel_gradf<R>(K, P.skf_intf, dist);
-                const R rhohi = pri.z, pi = pri.w;
+                const R rhohi = pri.z, pi = B.pres[i];
                 R pf = P.intf_approx == 1 ? (pjres + pi) / (rhohj * rhohi) : pi / (rhohi * rhohi) + pj_rho2;
                 if (P.art_pressure && (pi < R(0) || pjres < R(0))) {
                     const R fij = kernel_w<R>(K, P.skf_ap, dist) / (R)P.ap_wdelta;
@@ -417,41 +563,116 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
             }
         });
     }
-    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
-    if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
-        wave_max_bits<R>(&ctrl->amax_bits, live ? (R)sqrt((double)ax * ax + (double)ay * ay) : R(0));
-        wave_max_bits<R>(&ctrl->mumax_bits, live ? amu : R(0));
+    pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, avx, avy, amu, A,
+                                    key_next, der, xs, maxslots);
+}
+
+template <class R> __device__ __forceinline__ R fast_rcp(R x);
+template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return 1.0 / x; }
+template <class R> __device__ __forceinline__ R fast_rsqrt(R x);
+template <> __device__ __forceinline__ float fast_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> __device__ __forceinline__ double fast_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+
+// Pair pass for the common configuration: cubic spline for every term,
+// pressure in the p_i / rho_i^2 + p_j / rho_j^2 form without artificial
+// pressure, summation or plain continuity density.  Same physics as
+// pair_kernel, restructured for instruction count: one reciprocal square
+// root gives 1/|r| and |r|; W and grad W share q; pressure and artificial
+// viscosity share one coefficient on r_i - r_j; 1 / (rho_i + rho_j) serves
+// artificial viscosity and XSPH; the mass factors are applied once at the
+// end.  In FP32 the reciprocals are the hardware v_rcp / v_rsq (1 ulp).
+template <class R, bool FUSED, bool EXPORT>
+__global__ void __launch_bounds__(kBlock) pair_fast_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
+                                                           const uint32_t* __restrict__ skey,
+                                                           const uint32_t* __restrict__ LB, StateA<R> A,
+                                                           uint32_t* __restrict__ key_next,
+                                                           real4<R>* __restrict__ der, real4<R>* __restrict__ xs,
+                                                           unsigned long long* __restrict__ maxslots) {
+    if (ctrl->halted) return;
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    const bool live = s < P.n;
+    constexpr bool kF32 = sizeof(R) == 4;
+    const R h = (R)P.h;
+    const R inv_h = (R)(1.0 / P.h);
+    const R c_q1 = (R)(P.f_cubic / (P.h * P.h));  // grad W = diff * c_q1 (1.5 q - 2), q <= 1
+    const R c_q2 = (R)(0.5 * P.f_cubic / P.h);    // grad W = -diff * c_q2 (2 - q)^2 / |r|, 1 < q <= 2
+    const R w_q1 = (R)P.f_cubic;
+    const R w_q2 = (R)(P.f_cubic / 6.0);
+    const R alpha_c = (R)(P.av_alpha * P.av_c), beta = (R)P.av_beta, eta2 = (R)P.av_eta2;
+    const R lj_r0 = (R)P.lj_r0, lj_D = (R)P.lj_D;
+    const bool cont = P.density_treatment != 0;
+    R sx = R(0), sy = R(0), sd = R(0), xx = R(0), xy = R(0), amu = R(0), ljx = R(0), ljy = R(0);
+    bool overflow = false;
+    double2 pj = make_double2(0, 0);
+    real4<R> vaj{}, prj{};
+    uint32_t oid = 0;
+    if (live) {
+        pj = B.pos[s];
+        vaj = B.va[s];
+        prj = B.pred[s];
+        oid = B.oid[s];
     }
-    if (EXPORT || !FUSED) {
-        if (live) {
-            real4<R> d;
-            d.x = ax;
-            d.y = ay;
-            d.z = drho;
-            d.w = amu;
-            der[s] = d;
-            real4<R> x;
-            x.x = avx;
-            x.y = avy;
-            x.z = R(0);
-            x.w = R(0);
-            xs[s] = x;
-        }
+    const int tj = (int)vaj.w;
+    if (live && tj != 0) {
+        for_each_neighbour<kF32>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
+                                 [&](uint32_t i, double dxd, double dyd) {
+            const real4<R> vai = B.va[i];
+            const real4<R> pri = B.pred[i];
+            const R dx = (R)dxd, dy = (R)dyd;  // r_i - r_j
+            const R d2 = dx * dx + dy * dy;
+            // coincident particles (|r| = 0) contribute grad W = 0 as in the
+            // reference, where q = 0 selects the inner branch
+            const R rinv = d2 > R(0) ? fast_rsqrt<R>(d2) : R(0);
+            const R dist = d2 * rinv;
+            const R q = dist * inv_h;
+            const R t = R(2) - q;
+            R F, W;
+            if (q <= R(1)) {
+                F = c_q1 * (R(1.5) * q - R(2));
+                W = w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q);
+            } else {
+                F = -c_q2 * t * t * rinv;
+                W = w_q2 * t * t * t;
+            }
+            if (cont) sd += F * ((vai.x - vaj.x) * dx + (vai.y - vaj.y) * dy);
+            const int ti = (int)vai.w;
+            if (P.lj_on && tj > 0 && ti < 0 && dist < lj_r0) {
+                const R rn = lj_r0 / dist;  // |r| = 0 gives a non-finite force, as the reference
+                const R rn2 = rn * rn, rn4 = rn2 * rn2;
+                const R f = lj_D * (rn4 * rn4 * rn4 - rn4) / d2;
+                ljx -= dx * f;
+                ljy -= dy * f;
+            }
+            const R inv_rs = fast_rcp<R>(pri.z + prj.z);  // 1 / (rho^_i + rho^_j)
+            R coef = pri.w + prj.w;                         // p_i / rho_i^2 + p_j / rho_j^2
+            if (P.av_on) {
+                const R vr = (pri.x - prj.x) * dx + (pri.y - prj.y) * dy;  // (v_j - v_i) . (r_j - r_i)
+                if (vr < R(0)) {
+                    const R mu = h * vr * fast_rcp<R>(d2 + eta2);
+                    amu = fmax(amu, -mu);
+                    coef += (beta * mu - alpha_c) * mu * R(2) * inv_rs;
+                }
+            }
+            const R fc = F * coef;
+            sx += fc * dx;
+            sy += fc * dy;
+            if (P.xsph_on && ti > 0) {
+                const R cw = W * inv_rs;
+                xx += (pri.x - prj.x) * cw;
+                xy += (pri.y - prj.y) * cw;
+            }
+        });
     }
-    if (FUSED) {
-        uint32_t outside = 0, bad = 0;
-        if (lb == 0 && threadIdx.x == 0) {  // nobody reads these during the pass
-            ctrl->dt_new = ctrl->cur_dt_const;
-            ctrl->pending = 1;
-        }
-        if (live) {
-            const Ctrl& c = *ctrl;
-            integrate_one<R>(P, c, s, pj, vaj, prj, oid, ax, ay, drho, avx, avy, c.cur_dt_const, A, key_next,
-                             outside, bad);
-        }
-        wave_count_add(&ctrl->outside, outside);
-        if (bad) atomicOr(&ctrl->grid_overflow, 1u);
-    }
+    const R m = (R)P.mass;
+    const R ax = ljx + m * sx;
+    const R ay = -(R)P.g + ljy + m * sy;
+    const R drho = m * sd;
+    const R xf = m * R(2) * (R)P.xsph_eps;
+    pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, xf * xx,
+                                    xf * xy, amu, A, key_next, der, xs, maxslots);
 }
 
 // ------------------------------------------------------------------ integrate (dynamic dt)
@@ -460,12 +681,14 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
                                                            const real4<R>* __restrict__ der,
                                                            const real4<R>* __restrict__ xs, StateA<R> A,
-                                                           uint32_t* __restrict__ key_next) {
+                                                           uint32_t* __restrict__ key_next,
+                                                           const unsigned long long* __restrict__ maxslots) {
     Ctrl& c = *ctrl;
     if (c.halted) return;
     double dt;
     if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
-        const double amax = dbits(c.amax_bits), mumax = dbits(c.mumax_bits);
+        double amax, mumax;
+        slots_max2(maxslots, amax, mumax);
         dt = P.cfl * fmin(sqrt(P.h / amax), P.h / (P.c0 + mumax));
     } else {
         dt = c.cur_dt_const;
@@ -520,7 +743,7 @@ __global__ void init_kernel(StepParams P, const double* __restrict__ r, const do
 
 // Un-permute the state into original order (AoS doubles).
 template <class R>
-__global__ void download_kernel(StepParams P, StateA<R> A, const real4<R>* __restrict__ pred, double* __restrict__ r,
+__global__ void download_kernel(StepParams P, StateA<R> A, const R* __restrict__ pres, double* __restrict__ r,
                                 double* __restrict__ v, double* __restrict__ rho, double* __restrict__ p,
                                 int32_t* __restrict__ itype) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,7 +757,7 @@ __global__ void download_kernel(StepParams P, StateA<R> A, const real4<R>* __res
     v[2 * o + 1] = (double)va.y;
     rho[o] = (double)va.z;
     itype[o] = (int32_t)va.w;
-    p[o] = pred ? (double)pred[s].w : (double)eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
+    p[o] = pres ? (double)pres[s] : (double)eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
 }
 
 template <class R>
@@ -576,8 +799,8 @@ __global__ void neighbours_kernel(StepParams P, StateB<R> B, const uint32_t* __r
     uint32_t cnt = 0;
     bool overflow = false;
     if ((int)B.va[s].w != 0) {
-        for_each_neighbour(P, max_nbr - 1, s, skey[s], B.pos[s], B.pos, LB, overflow,
-                           [&](uint32_t i, double, double, double) {
+        for_each_neighbour<false>(P, max_nbr - 1, s, skey[s], B.pos[s], B.pos, LB, overflow,
+                                  [&](uint32_t i, double, double) {
             out[cnt + (size_t)max_nbr * o] = B.oid[i];
             ++cnt;
         });
