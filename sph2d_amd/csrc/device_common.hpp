@@ -102,6 +102,8 @@ struct StepParams {
     // launch geometry
     uint32_t nblocks;       // logical blocks of kBlock particles
     uint32_t blocks_per_xcd;
+    uint32_t ntiles;        // tiles of one wavefront (64 particles) for the tiled pair pass
+    uint32_t tile_blocks_per_xcd;
 };
 
 // XCD-aware block index: the hardware deals consecutive workgroups to the 8

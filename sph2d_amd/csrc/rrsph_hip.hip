@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,6 +18,7 @@
 #include "diagnostics.hpp"
 #include "radix_sort.hpp"
 #include "step_kernels.hpp"
+#include "pair_tile.hpp"
 
 using namespace rrsph;
 
@@ -89,6 +91,7 @@ struct rrsph_ctx {
     void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
          *pred_scratch = nullptr, *pres = nullptr, *pres_scratch = nullptr;
     bool fast_pair = false;
+    int pair_impl = 1;  // 0 general, 1 one-thread-per-particle batched global walk, 2 LDS tiles
     uint32_t *oidA = nullptr, *oidB = nullptr;
     uint32_t *key_next = nullptr, *kbuf[2] = {nullptr, nullptr}, *vbuf[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;
@@ -195,23 +198,28 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
     }
     real4<R>* der = (real4<R>*)c->der;
     real4<R>* xs = (real4<R>*)c->xs;
+    const uint32_t nt_xcd = P.tile_blocks_per_xcd * kNumXcd;
+    const int impl = c->fast_pair ? c->pair_impl : 0;
     auto pair = [&](auto kernel) {
+        const uint32_t grid = impl == 2 ? nt_xcd : nb_xcd;
+        const uint32_t block = impl == 2 ? kTile * kTilesPerBlock : kBlock;
         L(K_PAIR, [&] {
-            kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A, c->key_next, der, xs,
-                                                     c->maxslots);
+            kernel<<<grid, block, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A, c->key_next, der, xs,
+                                                  c->maxslots);
         });
     };
+    auto choose = [&](auto general, auto fast, auto tile) {
+        if (impl == 2) pair(tile);
+        else if (impl == 1) pair(fast);
+        else pair(general);
+    };
     if (c->fused) {
-        if (c->export_fields) {
-            if (c->fast_pair) pair(pair_fast_kernel<R, true, true>);
-            else pair(pair_kernel<R, true, true>);
-        } else {
-            if (c->fast_pair) pair(pair_fast_kernel<R, true, false>);
-            else pair(pair_kernel<R, true, false>);
-        }
+        if (c->export_fields)
+            choose(pair_kernel<R, true, true>, pair_fast_kernel<R, true, true>, pair_tile_kernel<R, true, true>);
+        else
+            choose(pair_kernel<R, true, false>, pair_fast_kernel<R, true, false>, pair_tile_kernel<R, true, false>);
     } else {
-        if (c->fast_pair) pair(pair_fast_kernel<R, false, true>);
-        else pair(pair_kernel<R, false, true>);
+        choose(pair_kernel<R, false, true>, pair_fast_kernel<R, false, true>, pair_tile_kernel<R, false, true>);
         L(K_INTEGRATE, [&] {
             integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots);
         });
@@ -382,6 +390,8 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     }
     P.nblocks = grid_blocks(P.n, kBlock);
     P.blocks_per_xcd = (P.nblocks + kNumXcd - 1) / kNumXcd;
+    P.ntiles = grid_blocks(P.n, kTile);
+    P.tile_blocks_per_xcd = (grid_blocks(P.ntiles, kTilesPerBlock) + kNumXcd - 1) / kNumXcd;
     // radix passes over the key bits (dead key = ncells must fit)
     uint32_t bits = 0;
     while ((1ull << bits) <= (uint64_t)P.ncells) ++bits;
@@ -390,6 +400,11 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     ctx->ntiles = grid_blocks(P.n, kSortTile);
     ctx->fp32 = p.precision == RRSPH_FP32;
     ctx->fused = !(p.dt_correction_method == 2 && p.dt_dynamic_mode == 1);
+    if (const char* e = std::getenv("RRSPH_HIP_PAIR_KERNEL")) {
+        if (!std::strcmp(e, "general")) ctx->pair_impl = 0;
+        else if (!std::strcmp(e, "tile")) ctx->pair_impl = 2;
+        else ctx->pair_impl = 1;
+    }
     ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&
                      p.average_velocity_skf == 1 && p.intf_sph_approximation == 2 && !p.artificial_pressure &&
                      (p.density_treatment == 0 || p.density_treatment == 1) && std::fabs(p.cell_scale_k - 2.0) == 0.0;
