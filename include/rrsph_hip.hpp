@@ -7,20 +7,21 @@
 //
 //   hip_time_integration(params, output, r_var, v_var, rho, p, itype)
 //     replaces time_integration(r_var, v_var, rho, p, itype)
-//     (RRSPH_OMP/include/TimeIntegration.h:4-9, TimeIntegration.cpp:21-103)
-//     and cl_time_integration (RRSPH_CL/include/CLAdapter.h:10-15).
+//     (sources/RRSPH_OMP/include/TimeIntegration.h:4-9, src/TimeIntegration.cpp:21-103)
+//     and cl_time_integration (sources/RRSPH_CL/include/CLAdapter.h:10-15).
 //
 // Requirements on the template arguments (all met by the reference's types):
-//   Params      : the fields of ExperimentParams (RRSPH_PARAMS/include/Params.h:8-194)
+//   Params      : the fields of ExperimentParams (sources/RRSPH_PARAMS/include/Params.h:8-194)
 //   Output      : setup_output(load_r, load_itype, load_v, load_p, load_rho),
 //                 start_step(t), update_step(t, step), finish_step()
-//                 (RRSPH_COMMON/include/Output.h:11-86)
+//                 (sources/RRSPH_COMMON/include/Output.h:11-31)
 //   FloatnArray : vheap_darray_floatn -- get_flt2() returning an indexable
 //                 array of {x, y}; constructible from a size
 //   FloatArray / IntArray : heap_darray<rr_float> / heap_darray<rr_int>
 // Errors are thrown as std::runtime_error with the reference's messages.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -45,9 +46,14 @@ struct Options {
     bool boundary_motion = false;
     // DT_CORRECTION_DYNAMIC: 0 = OpenMP build of RRSPH_OMP, 1 = max over particles
     int32_t dt_dynamic_mode = 0;
+    // nwm = 6 (Fourier piston, an extension of this library): piston
+    // velocity sum_k a_k cos(k w t) + b_k sin(k w t), at most 32 harmonics
+    double piston_omega = 0;
+    std::vector<double> piston_a, piston_b;
 };
 
-// c^2 and c of the EOS exactly as EOS.cpp:3-20 (RRSPH_OMP) computes them
+// c^2 and c of the EOS as the OpenMP backend computes them (eos_sound_vel_method 1
+// takes eos_sound_vel itself as c^2, sources/RRSPH_COMMON/src/EOS.cpp:3-20)
 template <class Params> inline void eos_constants(const Params& p, double& c_sqr, double& c) {
     c_sqr = p.eos_sound_vel_method == 0 ? 200 * p.g * p.depth * p.eos_sound_vel_coef : (double)p.eos_sound_vel;
     c = std::sqrt(c_sqr);
@@ -111,6 +117,12 @@ template <class Params> inline rrsph_params_c make_params(const Params& p, const
     c.nwm_phase = p.nwm_phase;
     c.boundary_motion = o.boundary_motion ? 1 : 0;
     c.dt_dynamic_mode = o.dt_dynamic_mode;
+    c.piston_nharm = (int32_t)std::min<size_t>(std::min(o.piston_a.size(), o.piston_b.size()), 32);
+    c.piston_omega = o.piston_omega;
+    for (int32_t k = 0; k < c.piston_nharm; ++k) {
+        c.piston_a[k] = o.piston_a[k];
+        c.piston_b[k] = o.piston_b[k];
+    }
     c.precision = (int32_t)o.precision;
     c.start_time = p.start_simulation_time;
     return c;

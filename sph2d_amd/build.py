@@ -48,5 +48,24 @@ def build(force: bool = True, verbose: bool = False) -> str:
     return OUT
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cpp")
+DRIVER_OUT = os.path.join(ROOT, "tests", "cpp", "dropin_driver")
+
+
+def build_dropin_driver() -> str:
+    """Host-only C++ program that drives the library through the drop-in
+    adapter include/rrsph_hip.hpp (g++, linked against librrsph_hip.so)."""
+    tmp = DRIVER_OUT + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", tmp, DRIVER_SRC,
+           "-L", HERE, "-lrrsph_hip", "-Wl,-rpath,$ORIGIN/../../sph2d_amd", "-Wl,-rpath-link,/opt/rocm/lib"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed for " + DRIVER_SRC)
+    os.replace(tmp, DRIVER_OUT)
+    return DRIVER_OUT
+
+
 if __name__ == "__main__":
     print(build(force=True, verbose="-v" in sys.argv))
+    print(build_dropin_driver())

@@ -440,11 +440,12 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     }
     ctx->prm = *params;
     *out = ctx;
+    // parameters are validated on the host before any device is touched
+    rrsph_status st = setup_params(ctx, *params);
+    if (st != RRSPH_OK) return st;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(ctx, RRSPH_ERR_NO_DEVICE, "RRSPH_HIP: no HIP device");
-    rrsph_status st = setup_params(ctx, *params);
-    if (st != RRSPH_OK) return st;
     HIP_TRY(hipGetDevice(&ctx->device));
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->own_stream = true;

@@ -321,7 +321,7 @@ class ExperimentParams:
     piston_omega: float = 0.0
     piston_a: tuple = ()
     piston_b: tuple = ()
-    # switches for RRSPH_CL-only behaviour (see DESIGN notes in the oracle header)
+    # switches for RRSPH_CL-only behaviour (described in the oracle header)
     boundary_motion: bool = False
     dt_dynamic_mode: int = 0
     pi: float = PI
@@ -334,7 +334,7 @@ class ExperimentParams:
 
     @property
     def eos_c_sqr(self) -> float:
-        """EOS.cpp:3-10 (RRSPH_OMP): method 1 uses eos_sound_vel itself as c^2."""
+        """RRSPH_COMMON/src/EOS.cpp:3-20: method 1 uses eos_sound_vel itself as c^2."""
         if self.eos_sound_vel_method == EOS_SOUND_VEL_DAM_BREAK:
             return 200 * self.g * self.depth * self.eos_sound_vel_coef
         return self.eos_sound_vel

@@ -79,3 +79,61 @@ def test_create_without_device_reports_error():
     with pytest.raises(H.RrsphError) as e:
         H.HipStep(lay.params)
     assert e.value.status == 8
+
+
+def _no_device():
+    try:
+        import torch
+
+        return not torch.cuda.is_available()
+    except ImportError:
+        return True
+
+
+@pytest.mark.parametrize(
+    "change,status,text",
+    [
+        ({"dim": 3}, 7, "only dim = 2"),
+        ({"ntotal": 0}, 1, "invalid sizes"),
+        ({"max_neighbours": 1}, 1, "invalid sizes"),
+        ({"x_maxgeom": 1.0e9}, 6, "can't make grid with so many cells"),
+        ({"nfluid": 10**7}, 1, "nfluid > ntotal"),
+    ],
+)
+def test_create_validates_params_on_host(change, status, text):
+    """Parameter errors are reported before any device is touched, with the
+    reference's message where it has one (Input.cpp:64-70 for the grid)."""
+    import copy
+
+    from sph2d_amd import generators as G
+
+    prm = copy.copy(G.dam_break().params)
+    for k, v in change.items():
+        setattr(prm, k, v)
+    with pytest.raises(H.RrsphError) as e:
+        H.HipStep(prm)
+    assert e.value.status == status
+    assert text in str(e.value)
+
+
+def test_dropin_driver_reports_errors_like_the_reference(tmp_path):
+    """The C++ drop-in (include/rrsph_hip.hpp, driven by tests/cpp/dropin_driver)
+    turns a library error into a std::exception caught in main, printed as
+    'Catch exception: <message>' (sources/SPH.cpp:76-85)."""
+    import dropin
+
+    from sph2d_amd import generators as G
+
+    if not os.path.exists(dropin.DRIVER):
+        pytest.fail("dropin_driver not built (python -m sph2d_amd.build)")
+    lay = G.dam_break()
+    prm = lay.params
+    prm.dim = 3
+    res = dropin.run(tmp_path, prm, lay, nsteps=1)
+    assert res.returncode == 1
+    assert "Catch exception: RRSPH_HIP: only dim = 2 is implemented" in res.stdout
+    if _no_device():
+        prm.dim = 2
+        res = dropin.run(tmp_path, prm, lay, nsteps=1)
+        assert res.returncode == 1
+        assert "Catch exception: RRSPH_HIP: no HIP device" in res.stdout
