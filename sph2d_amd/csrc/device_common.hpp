@@ -19,6 +19,16 @@ template <> struct V4<float> { using type = float4; };
 template <> struct V4<double> { using type = double4; };
 template <class R> using real4 = typename V4<R>::type;
 
+// |r|^2 = dx*dx + dy*dy with every product and sum rounded separately (no
+// fused multiply-add), the arithmetic of the reference's FP64 neighbour
+// test, so that the neighbour sets agree bit for bit.
+__device__ __forceinline__ double dist2_exact(double dx, double dy) {
+#pragma clang fp contract(off)
+    const double xx = dx * dx;
+    const double yy = dy * dy;
+    return xx + yy;
+}
+
 // Per-step control block in device memory.  The prologue (one thread, start
 // of each step) commits the previous step and snapshots the values every
 // other kernel of the step reads; kernels only read the snapshot.

@@ -122,7 +122,7 @@ __device__ __forceinline__ void pair_body(const PairConst<R>& C, PairAcc<R>& A, 
 __device__ __forceinline__ bool exact_inside(double2 pi, double2 pj, double cut2) {
     const double dx = __dsub_rn(pi.x, pj.x);
     const double dy = __dsub_rn(pi.y, pj.y);
-    return __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)) < cut2;
+    return dist2_exact(dx, dy) < cut2;
 }
 
 template <class R, bool FUSED, bool EXPORT>

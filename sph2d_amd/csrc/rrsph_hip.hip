@@ -91,7 +91,7 @@ struct rrsph_ctx {
     void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
          *pred_scratch = nullptr, *pres = nullptr, *pres_scratch = nullptr;
     bool fast_pair = false;
-    int pair_impl = 1;  // 0 general, 1 one-thread-per-particle batched global walk, 2 LDS tiles
+    int pair_impl = 1;  // 0 general, 1 two-phase walk with LDS hit lists, 2 LDS tiles, 3 single-phase walk
     uint32_t *oidA = nullptr, *oidB = nullptr;
     uint32_t *key_next = nullptr, *kbuf[2] = {nullptr, nullptr}, *vbuf[2] = {nullptr, nullptr};
     uint32_t* hist = nullptr;
@@ -208,18 +208,22 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
                                                   c->maxslots);
         });
     };
-    auto choose = [&](auto general, auto fast, auto tile) {
-        if (impl == 2) pair(tile);
+    auto choose = [&](auto general, auto fast, auto tile, auto walk) {
+        if (impl == 3) pair(walk);
+        else if (impl == 2) pair(tile);
         else if (impl == 1) pair(fast);
         else pair(general);
     };
     if (c->fused) {
         if (c->export_fields)
-            choose(pair_kernel<R, true, true>, pair_fast_kernel<R, true, true>, pair_tile_kernel<R, true, true>);
+            choose(pair_kernel<R, true, true>, pair_fast_kernel<R, true, true>, pair_tile_kernel<R, true, true>,
+                   pair_fast_kernel<R, true, true, false>);
         else
-            choose(pair_kernel<R, true, false>, pair_fast_kernel<R, true, false>, pair_tile_kernel<R, true, false>);
+            choose(pair_kernel<R, true, false>, pair_fast_kernel<R, true, false>, pair_tile_kernel<R, true, false>,
+                   pair_fast_kernel<R, true, false, false>);
     } else {
-        choose(pair_kernel<R, false, true>, pair_fast_kernel<R, false, true>, pair_tile_kernel<R, false, true>);
+        choose(pair_kernel<R, false, true>, pair_fast_kernel<R, false, true>, pair_tile_kernel<R, false, true>,
+               pair_fast_kernel<R, false, true, false>);
         L(K_INTEGRATE, [&] {
             integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots);
         });
@@ -403,6 +407,7 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     if (const char* e = std::getenv("RRSPH_HIP_PAIR_KERNEL")) {
         if (!std::strcmp(e, "general")) ctx->pair_impl = 0;
         else if (!std::strcmp(e, "tile")) ctx->pair_impl = 2;
+        else if (!std::strcmp(e, "walk")) ctx->pair_impl = 3;
         else ctx->pair_impl = 1;
     }
     ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&

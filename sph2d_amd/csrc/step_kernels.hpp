@@ -288,11 +288,11 @@ __device__ __forceinline__ void for_each_neighbour(const StepParams& P, uint32_t
                 const float f2 = fx * fx + fy * fy;
                 if (f2 >= cut_hi) continue;
                 if (f2 > cut_lo) {
-                    const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+                    const double d2 = dist2_exact(dx, dy);
                     if (!(d2 < P.cut2)) continue;
                 }
             } else {
-                const double d2 = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy));
+                const double d2 = dist2_exact(dx, dy);
                 if (!(d2 < P.cut2)) continue;
             }
             if (count == cap) {
@@ -344,9 +344,9 @@ __device__ __forceinline__ void for_each_neighbour_batched(const StepParams& P, 
                     const float fx = (float)dx, fy = (float)dy;
                     const float f2 = fx * fx + fy * fy;
                     in = f2 < cut_hi;
-                    if (in && f2 > cut_lo) in = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)) < P.cut2;
+                    if (in && f2 > cut_lo) in = dist2_exact(dx, dy) < P.cut2;
                 } else {
-                    in = __dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)) < P.cut2;
+                    in = dist2_exact(dx, dy) < P.cut2;
                 }
                 in = in && i < e && i != s;
                 mask |= (in ? 1u : 0u) << u;
@@ -365,6 +365,74 @@ __device__ __forceinline__ void for_each_neighbour_batched(const StepParams& P, 
             }
         }
     }
+}
+
+// Two-phase walk: each lane tests its candidates kCollect at a time
+// (independent position loads in flight, exact FP64 distance test without
+// branches) and appends the hits to its own column of an LDS list (`hits`,
+// stride kBlock, kChunk rows); whenever the list could overflow, and at the
+// end, flush(nh) consumes hits[0 .. nh).  The wave thereby runs the pair
+// body max_lanes(hits) times instead of once per candidate iteration in
+// which any lane hits.  Same neighbour set, visiting order and overflow
+// rule (first `cap` hits kept, the rest flagged) as for_each_neighbour.
+constexpr uint32_t kCollect = 4;
+template <uint32_t kChunk, class F>
+__device__ __forceinline__ void collect_neighbours(const StepParams& P, uint32_t cap, uint32_t s, uint32_t key,
+                                                   double2 pj, const double2* __restrict__ pos,
+                                                   const uint32_t* __restrict__ LB, uint32_t* __restrict__ hits,
+                                                   bool& overflow, F&& flush) {
+    static_assert(kChunk >= 2 * kCollect, "hit list too short");
+    const int cx = (int)(key / (uint32_t)P.ncy);
+    const int cy = (int)key - cx * P.ncy;
+    const int y0 = cy > 0 ? cy - 1 : 0;
+    const int y1 = cy + 1 < P.ncy ? cy + 1 : P.ncy - 1;
+    uint32_t nh = 0, done = 0;  // hits in the list, hits already flushed
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const int col = cx + c - 1;
+        if (col < 0 || col >= P.ncx) continue;
+        const uint32_t b = LB[col * P.ncy + y0];
+        const uint32_t e = LB[col * P.ncy + y1 + 1];
+#pragma unroll 1
+        for (uint32_t i = b; i < e; i += kCollect) {
+            double2 q[kCollect];
+#pragma unroll
+            for (uint32_t u = 0; u < kCollect; ++u) q[u] = pos[min(i + u, e - 1)];
+            bool hit[kCollect];
+#pragma unroll
+            for (uint32_t u = 0; u < kCollect; ++u) {
+                const double dx = __dsub_rn(q[u].x, pj.x);
+                const double dy = __dsub_rn(q[u].y, pj.y);
+                const double d2 = dist2_exact(dx, dy);
+                hit[u] = (i + u < e) & (i + u != s) & (d2 < P.cut2);
+            }
+            // nh <= kChunk - kCollect here, so every slot written is in range
+            if (done + nh + kCollect <= cap) {
+#pragma unroll
+                for (uint32_t u = 0; u < kCollect; ++u) {
+                    hits[nh * kBlock] = i + u;  // kept only when it is a hit
+                    nh += hit[u] ? 1u : 0u;
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < kCollect; ++u) {
+                    if (!hit[u]) continue;
+                    if (done + nh < cap) {
+                        hits[nh * kBlock] = i + u;
+                        ++nh;
+                    } else {
+                        overflow = true;
+                    }
+                }
+            }
+            if (nh + kCollect > kChunk) {
+                flush(nh);
+                done += nh;
+                nh = 0;
+            }
+        }
+    }
+    if (nh) flush(nh);
 }
 
 // ------------------------------------------------------------------ summation density
@@ -630,44 +698,115 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
                                     key_next, der, xs, maxslots);
 }
 
+// Reciprocal and reciprocal square root.  FP32: the hardware v_rcp / v_rsq
+// (1 ulp).  FP64: the hardware estimate refined by two Newton-Raphson steps
+// (y' = y + y (1 - x y) and y' = y + y (1 - x y^2) / 2), which recovers the
+// last bits of the double result.
 template <class R> __device__ __forceinline__ R fast_rcp(R x);
 template <> __device__ __forceinline__ float fast_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
-template <> __device__ __forceinline__ double fast_rcp<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ double fast_rcp<double>(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(y, fma(-x, y, 1.0), y);
+    y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
 template <class R> __device__ __forceinline__ R fast_rsqrt(R x);
 template <> __device__ __forceinline__ float fast_rsqrt<float>(float x) { return __builtin_amdgcn_rsqf(x); }
-template <> __device__ __forceinline__ double fast_rsqrt<double>(double x) { return 1.0 / sqrt(x); }
+template <> __device__ __forceinline__ double fast_rsqrt<double>(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    return y;
+}
 
 // Pair pass for the common configuration: cubic spline for every term,
 // pressure in the p_i / rho_i^2 + p_j / rho_j^2 form without artificial
 // pressure, summation or plain continuity density.  Same physics as
 // pair_kernel, restructured for instruction count: one reciprocal square
-// root gives 1/|r| and |r|; W and grad W share q; pressure and artificial
-// viscosity share one coefficient on r_i - r_j; 1 / (rho_i + rho_j) serves
-// artificial viscosity and XSPH; the mass factors are applied once at the
-// end.  In FP32 the reciprocals are the hardware v_rcp / v_rsq (1 ulp).
-template <class R, bool FUSED, bool EXPORT>
+// root gives 1/|r| and |r|; W and grad W share q and are selected without
+// branches; pressure and artificial viscosity share one coefficient on
+// r_i - r_j; 1 / (rho_i + rho_j) serves artificial viscosity and XSPH; the
+// mass factors are applied once at the end.
+template <class R> struct FastPair {
+    R h, inv_h, c_q1, c_q2, w_q1, w_q2, alpha_c, beta, eta2, lj_r0, lj_D;
+    bool cont, lj_on, av_on, xsph_on;
+    __device__ explicit FastPair(const StepParams& P)
+        : h((R)P.h), inv_h((R)(1.0 / P.h)),
+          c_q1((R)(P.f_cubic / (P.h * P.h))),   // grad W = diff * c_q1 (1.5 q - 2), q <= 1
+          c_q2((R)(0.5 * P.f_cubic / P.h)),     // grad W = -diff * c_q2 (2 - q)^2 / |r|, 1 < q <= 2
+          w_q1((R)P.f_cubic), w_q2((R)(P.f_cubic / 6.0)), alpha_c((R)(P.av_alpha * P.av_c)), beta((R)P.av_beta),
+          eta2((R)P.av_eta2), lj_r0((R)P.lj_r0), lj_D((R)P.lj_D), cont(P.density_treatment != 0),
+          lj_on(P.lj_on), av_on(P.av_on), xsph_on(P.xsph_on) {}
+};
+template <class R> struct PairSums {
+    R sx = 0, sy = 0, sd = 0, xx = 0, xy = 0, amu = 0, ljx = 0, ljy = 0;
+};
+
+// One neighbour i of particle j: dx, dy = r_i - r_j.
+template <class R>
+__device__ __forceinline__ void fast_pair_term(const FastPair<R>& K, PairSums<R>& S, R dx, R dy,
+                                               const real4<R>& vai, const real4<R>& pri, const real4<R>& vaj,
+                                               const real4<R>& prj, int tj) {
+    const R d2 = dx * dx + dy * dy;
+    // coincident particles (|r| = 0) contribute grad W = 0 as in the
+    // reference, where q = 0 selects the inner branch
+    const R rinv = d2 > R(0) ? fast_rsqrt<R>(d2) : R(0);
+    const R dist = d2 * rinv;
+    const R q = dist * K.inv_h;
+    const R t = R(2) - q;
+    const R t2 = t * t;
+    const bool inner = q <= R(1);
+    const R F = inner ? K.c_q1 * (R(1.5) * q - R(2)) : -K.c_q2 * t2 * rinv;
+    const R W = inner ? K.w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q) : K.w_q2 * t2 * t;
+    if (K.cont) S.sd += F * ((vai.x - vaj.x) * dx + (vai.y - vaj.y) * dy);
+    const int ti = (int)vai.w;
+    if (K.lj_on && tj > 0 && ti < 0 && dist < K.lj_r0) {
+        const R rn = K.lj_r0 / dist;  // |r| = 0 gives a non-finite force, as the reference
+        const R rn2 = rn * rn, rn4 = rn2 * rn2;
+        const R f = K.lj_D * (rn4 * rn4 * rn4 - rn4) / d2;
+        S.ljx -= dx * f;
+        S.ljy -= dy * f;
+    }
+    const R inv_rs = fast_rcp<R>(pri.z + prj.z);  // 1 / (rho^_i + rho^_j)
+    R coef = pri.w + prj.w;                         // p_i / rho_i^2 + p_j / rho_j^2
+    const R dvx = pri.x - prj.x, dvy = pri.y - prj.y;
+    if (K.av_on) {
+        const R vr = dvx * dx + dvy * dy;  // (v_j - v_i) . (r_j - r_i)
+        const R mu = K.h * vr * fast_rcp<R>(d2 + K.eta2);
+        const bool app = vr < R(0);
+        S.amu = app ? fmax(S.amu, -mu) : S.amu;
+        coef += app ? (K.beta * mu - K.alpha_c) * mu * R(2) * inv_rs : R(0);
+    }
+    const R fc = F * coef;
+    S.sx += fc * dx;
+    S.sy += fc * dy;
+    if (K.xsph_on) {
+        const R cw = ti > 0 ? W * inv_rs : R(0);
+        S.xx += dvx * cw;
+        S.xy += dvy * cw;
+    }
+}
+
+// LDS hit-list rows per lane of the two-phase walk (32 KiB per block)
+template <class R> constexpr uint32_t kPairChunk = 32u;
+
+template <class R, bool FUSED, bool EXPORT, bool COMPACT = true>
 __global__ void __launch_bounds__(kBlock) pair_fast_kernel(StepParams P, Ctrl* __restrict__ ctrl, StateB<R> B,
                                                            const uint32_t* __restrict__ skey,
                                                            const uint32_t* __restrict__ LB, StateA<R> A,
                                                            uint32_t* __restrict__ key_next,
                                                            real4<R>* __restrict__ der, real4<R>* __restrict__ xs,
                                                            unsigned long long* __restrict__ maxslots) {
+    __shared__ uint32_t hits[COMPACT ? kPairChunk<R> * kBlock : 1];
     if (ctrl->halted) return;
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
     const uint32_t s = lb * kBlock + threadIdx.x;
     const bool live = s < P.n;
     constexpr bool kF32 = sizeof(R) == 4;
-    const R h = (R)P.h;
-    const R inv_h = (R)(1.0 / P.h);
-    const R c_q1 = (R)(P.f_cubic / (P.h * P.h));  // grad W = diff * c_q1 (1.5 q - 2), q <= 1
-    const R c_q2 = (R)(0.5 * P.f_cubic / P.h);    // grad W = -diff * c_q2 (2 - q)^2 / |r|, 1 < q <= 2
-    const R w_q1 = (R)P.f_cubic;
-    const R w_q2 = (R)(P.f_cubic / 6.0);
-    const R alpha_c = (R)(P.av_alpha * P.av_c), beta = (R)P.av_beta, eta2 = (R)P.av_eta2;
-    const R lj_r0 = (R)P.lj_r0, lj_D = (R)P.lj_D;
-    const bool cont = P.density_treatment != 0;
-    R sx = R(0), sy = R(0), sd = R(0), xx = R(0), xy = R(0), amu = R(0), ljx = R(0), ljy = R(0);
+    const FastPair<R> K(P);
+    PairSums<R> S;
     bool overflow = false;
     double2 pj = make_double2(0, 0);
     real4<R> vaj{}, prj{};
@@ -680,67 +819,46 @@ __global__ void __launch_bounds__(kBlock) pair_fast_kernel(StepParams P, Ctrl* _
     }
     const int tj = (int)vaj.w;
     if (live && tj != 0) {
-        // FP32: one candidate at a time (fewer VGPRs, higher occupancy);
-        // FP64: batched candidate loads (measured faster on MI355X)
-        auto body = [&](uint32_t i, double dxd, double dyd) {
-            const real4<R> vai = B.va[i];
-            const real4<R> pri = B.pred[i];
-            const R dx = (R)dxd, dy = (R)dyd;  // r_i - r_j
-            const R d2 = dx * dx + dy * dy;
-            // coincident particles (|r| = 0) contribute grad W = 0 as in the
-            // reference, where q = 0 selects the inner branch
-            const R rinv = d2 > R(0) ? fast_rsqrt<R>(d2) : R(0);
-            const R dist = d2 * rinv;
-            const R q = dist * inv_h;
-            const R t = R(2) - q;
-            R F, W;
-            if (q <= R(1)) {
-                F = c_q1 * (R(1.5) * q - R(2));
-                W = w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q);
-            } else {
-                F = -c_q2 * t * t * rinv;
-                W = w_q2 * t * t * t;
-            }
-            if (cont) sd += F * ((vai.x - vaj.x) * dx + (vai.y - vaj.y) * dy);
-            const int ti = (int)vai.w;
-            if (P.lj_on && tj > 0 && ti < 0 && dist < lj_r0) {
-                const R rn = lj_r0 / dist;  // |r| = 0 gives a non-finite force, as the reference
-                const R rn2 = rn * rn, rn4 = rn2 * rn2;
-                const R f = lj_D * (rn4 * rn4 * rn4 - rn4) / d2;
-                ljx -= dx * f;
-                ljy -= dy * f;
-            }
-            const R inv_rs = fast_rcp<R>(pri.z + prj.z);  // 1 / (rho^_i + rho^_j)
-            R coef = pri.w + prj.w;                         // p_i / rho_i^2 + p_j / rho_j^2
-            if (P.av_on) {
-                const R vr = (pri.x - prj.x) * dx + (pri.y - prj.y) * dy;  // (v_j - v_i) . (r_j - r_i)
-                if (vr < R(0)) {
-                    const R mu = h * vr * fast_rcp<R>(d2 + eta2);
-                    amu = fmax(amu, -mu);
-                    coef += (beta * mu - alpha_c) * mu * R(2) * inv_rs;
+        if (COMPACT) {
+            uint32_t* my = hits + threadIdx.x;
+            // consume hits[0 .. nh): the loads of hit u + 1 are issued before
+            // the arithmetic of hit u, the list index of u + 2 before those
+            auto flush = [&](uint32_t nh) {
+                const uint32_t last = nh - 1;
+                uint32_t i1 = my[min(1u, last) * kBlock];
+                const uint32_t i0 = my[0];
+                double2 qn = B.pos[i0];
+                real4<R> van = B.va[i0], prn = B.pred[i0];
+#pragma unroll 1
+                for (uint32_t u = 0; u < nh; ++u) {
+                    const double2 q = qn;
+                    const real4<R> vai = van, pri = prn;
+                    qn = B.pos[i1];
+                    van = B.va[i1];
+                    prn = B.pred[i1];
+                    i1 = my[min(u + 2, last) * kBlock];
+                    fast_pair_term<R>(K, S, (R)__dsub_rn(q.x, pj.x), (R)__dsub_rn(q.y, pj.y), vai, pri, vaj, prj,
+                                      tj);
                 }
-            }
-            const R fc = F * coef;
-            sx += fc * dx;
-            sy += fc * dy;
-            if (P.xsph_on && ti > 0) {
-                const R cw = W * inv_rs;
-                xx += (pri.x - prj.x) * cw;
-                xy += (pri.y - prj.y) * cw;
-            }
-        };
-        if (kF32)
-            for_each_neighbour<true>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, body);
-        else
-            for_each_neighbour_batched<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, body);
+            };
+            collect_neighbours<kPairChunk<R>>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, my, overflow, flush);
+        } else {
+            auto body = [&](uint32_t i, double dxd, double dyd) {
+                fast_pair_term<R>(K, S, (R)dxd, (R)dyd, B.va[i], B.pred[i], vaj, prj, tj);
+            };
+            if (kF32)
+                for_each_neighbour<true>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, body);
+            else
+                for_each_neighbour_batched<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow, body);
+        }
     }
     const R m = (R)P.mass;
-    const R ax = ljx + m * sx;
-    const R ay = -(R)P.g + ljy + m * sy;
-    const R drho = m * sd;
+    const R ax = S.ljx + m * S.sx;
+    const R ay = -(R)P.g + S.ljy + m * S.sy;
+    const R drho = m * S.sd;
     const R xf = m * R(2) * (R)P.xsph_eps;
-    pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, xf * xx,
-                                    xf * xy, amu, A, key_next, der, xs, maxslots);
+    pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, xf * S.xx,
+                                    xf * S.xy, S.amu, A, key_next, der, xs, maxslots);
 }
 
 // ------------------------------------------------------------------ integrate (dynamic dt)

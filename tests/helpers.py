@@ -43,3 +43,53 @@ def rel_err(a, b, floor: float = 0.0) -> float:
     den = np.maximum(np.abs(b), floor)
     den = np.where(den == 0, 1, den)
     return float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+
+
+def near_cutoff_case(seed=7, centres=(12, 6), sats=64):
+    """Particles whose pair distances sit within a few ulps of the cut-off
+    k h: satellites on circles of radius k h around widely spaced centres.
+    Separate rounding of dx*dx + dy*dy and a fused multiply-add decide a
+    share of these pairs differently, so the neighbour sets pin the exact
+    FP64 arithmetic of the reference's test (GridFind.cpp:144-176)."""
+    import copy
+
+    from sph2d_amd import generators as G
+
+    lay = G.dam_break()
+    prm = copy.copy(lay.params)
+    cut = prm.cell_scale_k * prm.hsml
+    rng = np.random.default_rng(seed)
+    pts = []
+    for a in range(centres[0]):
+        for b in range(centres[1]):
+            c = np.array([0.2 + 3.1 * cut * a, 0.2 + 3.1 * cut * b]) + rng.uniform(0, 1e-3, 2)
+            pts.append(c)
+            th = rng.uniform(0, 2 * np.pi, sats)
+            for t in th:
+                pts.append(c + cut * np.array([np.cos(t), np.sin(t)]))
+    r = np.array(pts, dtype=np.float64)
+    n = len(r)
+    prm.ntotal = prm.nfluid = n
+    prm.nvirt = 0
+    prm.maxn = 1 << int(np.ceil(np.log2(n)) + 1)
+    prm.max_neighbours = 128
+    itype = np.full(n, 2, dtype=np.int32)
+    return prm, r, itype
+
+
+def brute_force_sets(r, cut2, itype):
+    """O(N^2) neighbour sets with separately rounded FP64 products (numpy
+    does not fuse), the definition of GridFind.cpp:144-176."""
+    n = len(r)
+    out = []
+    for j in range(n):
+        if itype[j] == 0:
+            out.append(np.zeros(0, dtype=np.uint32))
+            continue
+        dx = r[:, 0] - r[j, 0]
+        dy = r[:, 1] - r[j, 1]
+        d2 = dx * dx + dy * dy
+        m = (d2 < cut2) & (itype != 0)
+        m[j] = False
+        out.append(np.nonzero(m)[0].astype(np.uint32))
+    return out

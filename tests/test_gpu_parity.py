@@ -186,9 +186,9 @@ def test_general_pair_path_matches_oracle(variant):
     gpu.close()
 
 
-@pytest.mark.parametrize("impl", ["tile", "general"])
+@pytest.mark.parametrize("impl", ["tile", "general", "walk"])
 def test_alternative_pair_kernels_match_oracle(impl, monkeypatch):
-    """The LDS-tiled pair kernel and the general kernel (selected with
+    """The LDS-tiled pair kernel, the single-phase walk and the general kernel (selected with
     RRSPH_HIP_PAIR_KERNEL at context creation) give the oracle's result on
     the C0 dam break, FP64 and FP32."""
     monkeypatch.setenv("RRSPH_HIP_PAIR_KERNEL", impl)
@@ -205,3 +205,24 @@ def test_alternative_pair_kernels_match_oracle(impl, monkeypatch):
     g, r = gpu32.download(), ora.state()
     assert np.max(np.abs(g["rho"] - r["rho"]) / r["rho"]) < 1e-6
     gpu32.close()
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_neighbour_sets_near_cutoff(precision):
+    """Thousands of pairs within a few ulps of the cut-off (a fused
+    multiply-add in the distance test would flip ~1% of them): the GPU
+    neighbour sets equal the oracle's bit for bit."""
+    from helpers import near_cutoff_case
+
+    prm, r, itype = near_cutoff_case()
+    print("near-cutoff case n", prm.ntotal, "precision", precision, flush=True)
+    gpu = H.HipStep(prm, precision=precision)
+    v = np.zeros_like(r)
+    rho = np.full(prm.ntotal, prm.rho0)
+    gpu.upload(r, v, rho, itype)
+    nb_gpu = gpu.neighbours()
+    st, nb_ref = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    sg, sr = _sets(nb_gpu, prm.ntotal), _sets(nb_ref, prm.ntotal)
+    assert all(np.array_equal(a, b) for a, b in zip(sg, sr))
+    gpu.close()

@@ -166,3 +166,22 @@ def test_planted_mistake_is_caught(case):
     r, _v, rho, _it, NB = _ptrs(st, nb)
     oracle.lib().oracle_term_internal(ctypes.byref(op), r, NB, oracle.ptr(p), rho, oracle.ptr(out))
     assert not _close(out[:, 0], tg["OMP_intf2_artp_target__ax"], 6).all()
+
+
+def test_oracle_neighbour_sets_near_cutoff_match_brute_force():
+    """Pairs within a few ulps of the cut-off: the oracle's neighbour sets
+    equal an O(N^2) search with separately rounded products."""
+    from helpers import brute_force_sets, near_cutoff_case
+
+    prm, r, itype = near_cutoff_case()
+    st, nb = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    cut2 = (prm.cell_scale_k * prm.hsml) ** 2
+    got = oracle.neighbour_sets(nb, prm.ntotal)
+    want = brute_force_sets(r, cut2, itype)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    # the case is sharp: a fused multiply-add would decide some pairs differently
+    dx = r[:, None, 0] - r[None, :, 0]
+    dy = r[:, None, 1] - r[None, :, 1]
+    near = np.abs(dx * dx + dy * dy - cut2) < 4 * np.spacing(cut2)
+    assert near.sum() > 100
