@@ -203,6 +203,14 @@ const char* rrsph_hip_kernel_class_name(uint32_t k);
  * per-launch events); returns milliseconds. */
 rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total_ms);
 
+/* Cell-sort statistics.  No reference counterpart (the reference re-sorts
+ * from scratch every step, GridFind.cpp:47-88 / CLTimeIntegration.cpp:139-210):
+ * steps whose order came from the incremental re-sort, from the full radix
+ * sort, re-runs after a step changed more cells than the incremental path
+ * takes, and the number of particles that changed cell in the last step. */
+rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, uint64_t* full_steps,
+                                  uint64_t* fallbacks, uint32_t* last_changed);
+
 /* Version hooks of the backend (RRSPH_COMMON/include/CommonIncl.h:14-17). */
 uint32_t RRSPH_HIP_GetSpecificVersionMajor(void);
 uint32_t RRSPH_HIP_GetSpecificVersionMinor(void);

@@ -49,7 +49,7 @@ struct Ctrl {
     uint32_t outside;       // particles outside the geometry after whole_step
     uint32_t nbr_overflow;  // a particle had more than max_neighbours - 1 neighbours
     uint32_t grid_overflow; // a live particle lay outside the cell grid
-    uint32_t pad0;
+    uint32_t resort_count;  // slots that changed cell key since the last sort
     // snapshot of the step in flight (written by the prologue)
     double cur_time;
     double cur_dt;          // dt of predict_half_step
@@ -114,7 +114,13 @@ struct StepParams {
     uint32_t blocks_per_xcd;
     uint32_t ntiles;        // tiles of one wavefront (64 particles) for the tiled pair pass
     uint32_t tile_blocks_per_xcd;
+    // incremental re-sort: per kResortTile slots, the number of particles the
+    // integrator moved to another cell (zeroed by the reorder of each step)
+    uint32_t* resort_cnt;
+    uint32_t resort_tiles;
 };
+
+constexpr uint32_t kResortTile = 1024;  // slots per re-sort tile (a multiple of kBlock)
 
 // XCD-aware block index: the hardware deals consecutive workgroups to the 8
 // XCDs round-robin; this mapping gives each XCD one contiguous slice of the

@@ -346,7 +346,7 @@ __global__ void __launch_bounds__(kTile * kTilesPerBlock) pair_tile_kernel(StepP
     const R drho = m * acc.sd;
     const R xf = m * R(2) * (R)P.xsph_eps;
     pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, xf * acc.xx,
-                                    xf * acc.xy, acc.amu, A, key_next, der, xs, maxslots);
+                                    xf * acc.xy, acc.amu, A, key_next, live ? skey[s] : 0u, der, xs, maxslots);
 }
 
 }  // namespace rrsph

@@ -17,6 +17,7 @@
 #include "device_common.hpp"
 #include "diagnostics.hpp"
 #include "radix_sort.hpp"
+#include "resort.hpp"
 #include "step_kernels.hpp"
 #include "pair_tile.hpp"
 
@@ -33,11 +34,15 @@ enum KernelClass : uint32_t {
     K_DENSITY_SUM,
     K_PAIR,
     K_INTEGRATE,
+    K_RESORT_COMPACT,
+    K_RESORT_SORT,
+    K_RESORT_SCATTER,
     K_NUM_CLASSES
 };
 const char* const kClassNames[K_NUM_CLASSES] = {"step_prologue", "radix_upsweep", "radix_scan", "radix_downsweep",
                                                 "reorder_kernel", "density_sum_kernel", "pair_kernel",
-                                                "integrate_kernel"};
+                                                "integrate_kernel", "resort_compact", "resort_sort_small",
+                                                "resort_scatter"};
 
 struct Profiler {
     std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> rec;
@@ -107,10 +112,22 @@ struct rrsph_ctx {
            *st_drho = nullptr, *st_av = nullptr, *st_mu = nullptr;
     int32_t* st_itype = nullptr;
     uint32_t* st_u32 = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;        // step with the incremental re-sort
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph_full = nullptr;   // step with the full radix sort
+    hipGraphExec_t gexec_full = nullptr;
     const uint32_t* sorted_keys = nullptr;
     const uint32_t* sorted_perm = nullptr;
+    // incremental re-sort
+    uint32_t* skeyA = nullptr;          // key each slot of A was sorted by
+    uint32_t* chg_before = nullptr;     // changed slots before s (n + 1 entries)
+    uint32_t* rtile = nullptr;          // per-tile changed counts (filled by the integrator)
+    unsigned long long* chg = nullptr;  // changed slots as (key << 32 | slot)
+    uint32_t rtiles = 0;
+    uint32_t resort_cap = kResortCap;
+    bool incremental = true;            // false: every step runs the full radix sort
+    bool need_full = true;              // skeyA does not describe A (after upload)
+    uint64_t n_incremental = 0, n_full = 0, n_fallback = 0;
 };
 
 namespace {
@@ -178,18 +195,39 @@ void enqueue_sort(rrsph_ctx* c, Ctrl* ctrl, Launcher& L) {
     c->sorted_perm = src_v;
 }
 
-template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
+// Incremental re-sort of key_next against skeyA into kbuf[0] / vbuf[0].
+void enqueue_resort(rrsph_ctx* c, Launcher& L) {
+    const uint32_t n = c->P.n;
+    const uint32_t nb = grid_blocks(n, kBlock);
+    L(K_RESORT_COMPACT, [&] {
+        resort_compact<<<c->rtiles, kResortBlock, 0, c->stream>>>(c->key_next, c->skeyA, n, c->rtile, c->resort_cap,
+                                                                  c->chg_before, c->chg, c->ctrl);
+    });
+    L(K_RESORT_SORT, [&] {
+        resort_sort_small<<<1, kResortBlock, 0, c->stream>>>(c->rtile, c->rtiles, c->chg_before, n, c->resort_cap,
+                                                             c->chg, c->ctrl);
+    });
+    L(K_RESORT_SCATTER, [&] {
+        resort_scatter<<<nb, kBlock, 0, c->stream>>>(c->key_next, c->skeyA, c->chg_before, c->chg, n, c->kbuf[0],
+                                                     c->vbuf[0], c->ctrl);
+    });
+    c->sorted_keys = c->kbuf[0];
+    c->sorted_perm = c->vbuf[0];
+}
+
+template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof, bool full_sort) {
     Launcher L{c, prof};
     const StepParams& P = c->P;
     const uint32_t nb = grid_blocks(P.n, kBlock);
     const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
     L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
-    enqueue_sort(c, c->ctrl, L);
+    if (full_sort) enqueue_sort(c, c->ctrl, L);
+    else enqueue_resort(c, L);
     StateA<R> A = stateA<R>(c);
     StateB<R> B = stateB<R>(c, c->pred, c->pres);
     L(K_REORDER, [&] {
         reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
-                                                        c->maxslots);
+                                                        c->maxslots, c->skeyA);
     });
     if (P.density_treatment == 0) {
         L(K_DENSITY_SUM, [&] {
@@ -225,29 +263,39 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof) {
         choose(pair_kernel<R, false, true>, pair_fast_kernel<R, false, true>, pair_tile_kernel<R, false, true>,
                pair_fast_kernel<R, false, true, false>);
         L(K_INTEGRATE, [&] {
-            integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots);
+            integrate_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots,
+                                                              c->sorted_keys);
         });
     }
 }
 
-void enqueue_step(rrsph_ctx* c, Profiler* prof) {
-    if (c->fp32) enqueue_step_t<float>(c, prof);
-    else enqueue_step_t<double>(c, prof);
+void enqueue_step(rrsph_ctx* c, Profiler* prof, bool full_sort) {
+    if (c->fp32) enqueue_step_t<float>(c, prof, full_sort);
+    else enqueue_step_t<double>(c, prof, full_sort);
 }
 
 void drop_graph(rrsph_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->gexec_full) (void)hipGraphExecDestroy(c->gexec_full);
+    if (c->graph_full) (void)hipGraphDestroy(c->graph_full);
     c->gexec = nullptr;
     c->graph = nullptr;
+    c->gexec_full = nullptr;
+    c->graph_full = nullptr;
 }
 
+// Capture one step with the incremental re-sort and one with the full sort.
 rrsph_status build_graph(rrsph_ctx* ctx) {
     drop_graph(ctx);
     HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(ctx, nullptr);
+    enqueue_step(ctx, nullptr, false);
     HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph));
     HIP_TRY(hipGraphInstantiate(&ctx->gexec, ctx->graph, nullptr, nullptr, 0));
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(ctx, nullptr, true);
+    HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph_full));
+    HIP_TRY(hipGraphInstantiate(&ctx->gexec_full, ctx->graph_full, nullptr, nullptr, 0));
     return RRSPH_OK;
 }
 
@@ -260,7 +308,7 @@ template <class R> void enqueue_util_sort(rrsph_ctx* c) {
     StateA<R> A = stateA<R>(c);
     StateB<R> B = stateB<R>(c, c->pred_scratch, c->pres_scratch);
     reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
-                                                    c->maxslots + 2 * kMaxSlots);
+                                                    c->maxslots + 2 * kMaxSlots, nullptr);
 }
 
 std::string status_message(uint32_t st) {
@@ -277,7 +325,7 @@ rrsph_status sync_status(rrsph_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const Ctrl& c = *ctx->host_ctrl;
-    if (c.halted && c.status != 0) {
+    if (c.halted && c.status != 0 && c.status != kStatusResort) {
         return fail(ctx, (rrsph_status)c.status, status_message(c.status));
     }
     if (c.pending) {
@@ -403,6 +451,12 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     if (ctx->passes == 0) ctx->passes = 1;
     ctx->ntiles = grid_blocks(P.n, kSortTile);
     ctx->fp32 = p.precision == RRSPH_FP32;
+    ctx->rtiles = grid_blocks(P.n, kResortTile);
+    if (const char* e = std::getenv("RRSPH_HIP_SORT")) ctx->incremental = std::strcmp(e, "full") != 0;
+    if (const char* e = std::getenv("RRSPH_HIP_RESORT_CAP")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 0 && v < (long)kResortCap) ctx->resort_cap = (uint32_t)v;
+    }
     ctx->fused = !(p.dt_correction_method == 2 && p.dt_dynamic_mode == 1);
     if (const char* e = std::getenv("RRSPH_HIP_PAIR_KERNEL")) {
         if (!std::strcmp(e, "general")) ctx->pair_impl = 0;
@@ -423,7 +477,7 @@ void free_all(rrsph_ctx* c) {
                     c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->digit_total,
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
-                    c->st_mu, c->st_itype, c->st_u32};
+                    c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg_before, c->rtile, c->chg};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -482,6 +536,12 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->ctrl = dalloc<Ctrl>(1, e);
     ctx->util_ctrl = dalloc<Ctrl>(1, e);
     ctx->red = dalloc<double>(kDiagSlots * (size_t)kDiagMaxBlocks + 64, e);
+    ctx->skeyA = dalloc<uint32_t>(n, e);
+    ctx->chg_before = dalloc<uint32_t>(n + 1, e);
+    ctx->rtile = dalloc<uint32_t>(ctx->rtiles + 1, e);
+    ctx->P.resort_cnt = ctx->rtile;
+    ctx->P.resort_tiles = ctx->rtiles;
+    ctx->chg = dalloc<unsigned long long>(kResortCap, e);
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     std::memset(ctx->host_ctrl, 0, sizeof(Ctrl));
@@ -546,29 +606,64 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->uploaded = true;
     ctx->stepped = false;
+    ctx->need_full = true;
     return RRSPH_OK;
 }
 
+// Enqueue nsteps steps: the first after an upload (or a re-sort overflow)
+// with the full radix sort, the others with the incremental re-sort.
 static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
-    if (ctx->use_graph) {
-        if (!ctx->gexec) {
-            rrsph_status st = build_graph(ctx);
-            if (st != RRSPH_OK) return st;
-        }
-        for (uint32_t k = 0; k < nsteps; ++k) HIP_TRY(hipGraphLaunch(ctx->gexec, ctx->stream));
-    } else {
-        for (uint32_t k = 0; k < nsteps; ++k) enqueue_step(ctx, nullptr);
-        HIP_TRY(hipGetLastError());
+    if (ctx->use_graph && !ctx->gexec) {
+        rrsph_status st = build_graph(ctx);
+        if (st != RRSPH_OK) return st;
     }
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        const bool full = ctx->need_full || !ctx->incremental;
+        if (ctx->use_graph) HIP_TRY(hipGraphLaunch(full ? ctx->gexec_full : ctx->gexec, ctx->stream));
+        else enqueue_step(ctx, nullptr, full);
+        ++(full ? ctx->n_full : ctx->n_incremental);
+        ctx->need_full = false;
+    }
+    HIP_TRY(hipGetLastError());
     if (nsteps) ctx->stepped = true;
     return RRSPH_OK;
 }
 
+static uint64_t committed_steps(const Ctrl& c) { return c.step + (c.pending ? 1 : 0); }
+
+// After a synchronised launch: if a step found more changed slots than the
+// incremental re-sort takes, it halted before touching the state; clear the
+// halt and re-run it (and the steps after it) starting with the full sort.
+static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t nsteps, uint32_t* left) {
+    Ctrl& c = *ctx->host_ctrl;
+    *left = 0;
+    if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
+    ++ctx->n_fallback;
+    *left = nsteps - (uint32_t)(committed_steps(c) - first);
+    c.halted = 0;
+    c.status = 0;
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    ctx->need_full = true;
+    return RRSPH_OK;
+}
+
+// Run nsteps steps to completion (or to a halt) and report the status.
+static rrsph_status run_steps(rrsph_ctx* ctx, uint32_t nsteps) {
+    const uint64_t first = committed_steps(*ctx->host_ctrl);
+    uint32_t left = nsteps;
+    while (true) {
+        rrsph_status st = launch_steps(ctx, left);
+        if (st != RRSPH_OK) return st;
+        st = sync_status(ctx);
+        if (st != RRSPH_OK) return st;
+        st = resume_after_resort(ctx, first, nsteps, &left);
+        if (st != RRSPH_OK || !left) return st;
+    }
+}
+
 rrsph_status rrsph_hip_step(rrsph_ctx* ctx, uint32_t nsteps) {
     if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
-    rrsph_status st = launch_steps(ctx, nsteps);
-    if (st != RRSPH_OK) return st;
-    return sync_status(ctx);
+    return run_steps(ctx, nsteps);
 }
 
 rrsph_status rrsph_hip_run_until(rrsph_ctx* ctx, double t_end, uint64_t max_steps, uint64_t* steps_taken) {
@@ -584,9 +679,7 @@ rrsph_status rrsph_hip_run_until(rrsph_ctx* ctx, double t_end, uint64_t max_step
     rrsph_status st = RRSPH_OK;
     while (done < max_steps) {
         const uint32_t chunk = (uint32_t)std::min<uint64_t>(max_steps - done, 256);
-        st = launch_steps(ctx, chunk);
-        if (st != RRSPH_OK) break;
-        st = sync_status(ctx);
+        st = run_steps(ctx, chunk);
         done += chunk;
         if (st != RRSPH_OK || ctx->host_ctrl->halted) break;
     }
@@ -790,7 +883,13 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, ctx->stream));
-    for (uint32_t k = 0; k < nsteps; ++k) enqueue_step(ctx, &prof);
+    const uint64_t first = committed_steps(*ctx->host_ctrl);
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        const bool full = ctx->need_full || !ctx->incremental;
+        enqueue_step(ctx, &prof, full);
+        ++(full ? ctx->n_full : ctx->n_incremental);
+        ctx->need_full = false;
+    }
     HIP_TRY(hipEventRecord(t1, ctx->stream));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -811,7 +910,12 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     if (total_ms) *total_ms = tot;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    return sync_status(ctx);
+    rrsph_status st = sync_status(ctx);
+    if (st != RRSPH_OK) return st;
+    uint32_t left = 0;
+    st = resume_after_resort(ctx, first, nsteps, &left);
+    if (st != RRSPH_OK || !left) return st;
+    return run_steps(ctx, left);  // finish the steps after a re-sort overflow, unprofiled
 }
 
 rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total_ms) {
@@ -824,7 +928,7 @@ rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
     HIP_TRY(hipEventRecord(t0, ctx->stream));
-    rrsph_status st = launch_steps(ctx, nsteps);
+    rrsph_status st = run_steps(ctx, nsteps);
     HIP_TRY(hipEventRecord(t1, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     float ms = 0;
@@ -832,8 +936,17 @@ rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total
     if (total_ms) *total_ms = ms;
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    if (st != RRSPH_OK) return st;
-    return sync_status(ctx);
+    return st;
+}
+
+rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, uint64_t* full_steps,
+                                  uint64_t* fallbacks, uint32_t* last_changed) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    if (incremental_steps) *incremental_steps = ctx->n_incremental;
+    if (full_steps) *full_steps = ctx->n_full;
+    if (fallbacks) *fallbacks = ctx->n_fallback;
+    if (last_changed) *last_changed = ctx->host_ctrl ? ctx->host_ctrl->resort_count : 0;
+    return RRSPH_OK;
 }
 
 uint32_t RRSPH_HIP_GetSpecificVersionMajor(void) { return RRSPH_HIP_VERSION_MAJOR; }

@@ -187,23 +187,29 @@ __global__ void step_prologue(StepParams P, Ctrl* ctrl) {
 // ------------------------------------------------------------------ reorder
 // Gather the state into sorted order, predict (PredictHalfStep.cpp:16-22),
 // compute the pressure of the current density (Density.h:112) and fill the
-// cell-start table: LB[c] = first sorted index whose key >= c.
+// cell-start table: LB[c] = first sorted index whose key >= c.  With
+// skey_keep, also keep the sorted keys (the next step's incremental re-sort
+// compares them with the keys after integration).
 template <class R>
 __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
                                                          const uint32_t* __restrict__ skey,
                                                          const uint32_t* __restrict__ perm, StateA<R> A, StateB<R> B,
                                                          uint32_t* __restrict__ LB,
-                                                         unsigned long long* __restrict__ maxslots) {
+                                                         unsigned long long* __restrict__ maxslots,
+                                                         uint32_t* __restrict__ skey_keep) {
     if (ctrl->halted) return;
     if (blockIdx.x == 0) {  // clear the max slots of this step's reductions
         maxslots[threadIdx.x] = 0;
         maxslots[kMaxSlots + threadIdx.x] = 0;
+        // and the changed-cell counts the integrator of this step fills
+        if (skey_keep)
+            for (uint32_t t = threadIdx.x; t < P.resort_tiles; t += kBlock) P.resort_cnt[t] = 0;
     }
     const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
     const bool live = s0 < P.n;
     const uint32_t s = live ? s0 : P.n - 1;  // idle lanes shadow the last particle (no writes)
     const uint32_t k = skey[s];
-    const uint32_t src = perm[s];
+    const uint32_t src = min(perm[s], P.n - 1);  // a permutation of [0, n); the clamp only guards memory
     const double2 pos = A.pos[src];
     const real4<R> va = A.va[src];
     const real4<R> ad = A.ad[src];
@@ -212,6 +218,7 @@ __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctr
         B.pos[s] = pos;
         B.va[s] = va;
         B.oid[s] = oid;
+        if (skey_keep) skey_keep[s] = k;  // the key slot s is sorted by, for the next re-sort
     }
     const R dt = (R)ctrl->cur_dt;
     const bool cont = P.density_treatment != 0;
@@ -475,7 +482,8 @@ template <class R>
 __device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c, uint32_t s, double2 pos,
                                               real4<R> va, const real4<R>& pred, uint32_t oid, R ax, R ay, R drho,
                                               R avx, R avy, double dt, StateA<R> A, uint32_t* __restrict__ key_next,
-                                              uint32_t& outside, uint32_t& bad) {
+                                              uint32_t key_old, uint32_t& outside, uint32_t& bad,
+                                              uint32_t& changed) {
     const double vdt_d = c.cur_first ? dt * 0.5 : dt;
     const R vdt = (R)vdt_d;
     int itype = (int)va.w;
@@ -526,7 +534,9 @@ __device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c
     nad.w = R(0);
     A.ad[s] = nad;
     A.oid[s] = oid;
-    key_next[s] = itype == 0 ? P.ncells : cell_key(P, pos.x, pos.y, bad);
+    const uint32_t key = itype == 0 ? P.ncells : cell_key(P, pos.x, pos.y, bad);
+    key_next[s] = key;
+    changed = key != key_old ? 1u : 0u;
 }
 
 __device__ __forceinline__ void wave_count_add(uint32_t* dst, uint32_t v) {
@@ -545,7 +555,7 @@ template <class R, bool FUSED, bool EXPORT>
 __device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restrict__ ctrl, uint32_t lb, uint32_t s,
                                               bool live, bool overflow, double2 pj, const real4<R>& vaj,
                                               const real4<R>& prj, uint32_t oid, R ax, R ay, R drho, R avx, R avy,
-                                              R amu, StateA<R> A, uint32_t* __restrict__ key_next,
+                                              R amu, StateA<R> A, uint32_t* __restrict__ key_next, uint32_t key_old,
                                               real4<R>* __restrict__ der, real4<R>* __restrict__ xs,
                                               unsigned long long* __restrict__ maxslots) {
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
@@ -570,7 +580,7 @@ __device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restr
         }
     }
     if (FUSED) {
-        uint32_t outside = 0, bad = 0;
+        uint32_t outside = 0, bad = 0, changed = 0;
         if (lb == 0 && threadIdx.x == 0) {  // nobody reads these during the pass
             ctrl->dt_new = ctrl->cur_dt_const;
             ctrl->pending = 1;
@@ -578,9 +588,11 @@ __device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restr
         if (live) {
             const Ctrl& c = *ctrl;
             integrate_one<R>(P, c, s, pj, vaj, prj, oid, ax, ay, drho, avx, avy, c.cur_dt_const, A, key_next,
-                             outside, bad);
+                             key_old, outside, bad, changed);
         }
         wave_count_add(&ctrl->outside, outside);
+        // a wave's slots lie in one re-sort tile; lanes past n count nothing
+        wave_count_add(&P.resort_cnt[s / kResortTile], changed);
         if (bad) atomicOr(&ctrl->grid_overflow, 1u);
     }
 }
@@ -695,7 +707,7 @@ __global__ void __launch_bounds__(kBlock) pair_kernel(StepParams P, Ctrl* __rest
         });
     }
     pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, avx, avy, amu, A,
-                                    key_next, der, xs, maxslots);
+                                    key_next, live ? skey[s] : 0u, der, xs, maxslots);
 }
 
 // Reciprocal and reciprocal square root.  FP32: the hardware v_rcp / v_rsq
@@ -858,7 +870,7 @@ __global__ void __launch_bounds__(kBlock) pair_fast_kernel(StepParams P, Ctrl* _
     const R drho = m * S.sd;
     const R xf = m * R(2) * (R)P.xsph_eps;
     pair_epilogue<R, FUSED, EXPORT>(P, ctrl, lb, s, live, overflow, pj, vaj, prj, oid, ax, ay, drho, xf * S.xx,
-                                    xf * S.xy, S.amu, A, key_next, der, xs, maxslots);
+                                    xf * S.xy, S.amu, A, key_next, live ? skey[s] : 0u, der, xs, maxslots);
 }
 
 // ------------------------------------------------------------------ integrate (dynamic dt)
@@ -868,7 +880,8 @@ __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* _
                                                            const real4<R>* __restrict__ der,
                                                            const real4<R>* __restrict__ xs, StateA<R> A,
                                                            uint32_t* __restrict__ key_next,
-                                                           const unsigned long long* __restrict__ maxslots) {
+                                                           const unsigned long long* __restrict__ maxslots,
+                                                           const uint32_t* __restrict__ skey) {
     Ctrl& c = *ctrl;
     if (c.halted) return;
     double dt;
@@ -889,14 +902,15 @@ __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* _
         return;
     }
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    uint32_t outside = 0, bad = 0;
+    uint32_t outside = 0, bad = 0, changed = 0;
     if (s < P.n) {
         const real4<R> d = der[s];
         const real4<R> x = xs[s];
         integrate_one<R>(P, c, s, B.pos[s], B.va[s], B.pred[s], B.oid[s], d.x, d.y, d.z, x.x, x.y, dt, A, key_next,
-                         outside, bad);
+                         skey[s], outside, bad, changed);
     }
     wave_count_add(&c.outside, outside);
+    wave_count_add(&P.resort_cnt[s / kResortTile], changed);
     if (bad) atomicOr(&c.grid_overflow, 1u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.dt_new = dt;

@@ -37,7 +37,8 @@ EXPORTED = [
     "rrsph_hip_run_until", "rrsph_hip_download", "rrsph_hip_get_time", "rrsph_hip_set_export_fields",
     "rrsph_hip_fields", "rrsph_hip_cell_keys", "rrsph_hip_neighbours", "rrsph_hip_diagnostics", "rrsph_hip_gauges",
     "rrsph_hip_stream", "rrsph_hip_set_graph", "rrsph_hip_profile_steps", "rrsph_hip_kernel_class_count",
-    "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "RRSPH_HIP_GetSpecificVersionMajor",
+    "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats",
+    "RRSPH_HIP_GetSpecificVersionMajor",
     "RRSPH_HIP_GetSpecificVersionMinor", "RRSPH_HIP_GetSpecificVersionPatch", "RRSPH_HIP_GetSpecificVersionName",
 ]
 
@@ -130,6 +131,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "rrsph_hip_kernel_class_count": (ctypes.c_uint32, []),
         "rrsph_hip_kernel_class_name": (ctypes.c_char_p, [ctypes.c_uint32]),
         "rrsph_hip_time_steps": (st, [vp, ctypes.c_uint32, dp]),
+        "rrsph_hip_sort_stats": (st, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64), up]),
         "RRSPH_HIP_GetSpecificVersionMajor": (ctypes.c_uint32, []),
         "RRSPH_HIP_GetSpecificVersionMinor": (ctypes.c_uint32, []),
         "RRSPH_HIP_GetSpecificVersionPatch": (ctypes.c_uint32, []),
@@ -269,6 +272,15 @@ class HipStep:
         ms = ctypes.c_double()
         self._check(self.L.rrsph_hip_time_steps(self.ctx, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def sort_stats(self) -> Dict[str, int]:
+        """Steps sorted incrementally / fully, re-sort fallbacks, particles
+        that changed cell in the last step."""
+        inc, full, fb = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        last = ctypes.c_uint32()
+        self._check(self.L.rrsph_hip_sort_stats(self.ctx, ctypes.byref(inc), ctypes.byref(full), ctypes.byref(fb),
+                                                ctypes.byref(last)))
+        return {"incremental": inc.value, "full": full.value, "fallbacks": fb.value, "last_changed": last.value}
 
     def profile_steps(self, n: int):
         k = self.L.rrsph_hip_kernel_class_count()

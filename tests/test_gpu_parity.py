@@ -226,3 +226,38 @@ def test_neighbour_sets_near_cutoff(precision):
     sg, sr = _sets(nb_gpu, prm.ntotal), _sets(nb_ref, prm.ntotal)
     assert all(np.array_equal(a, b) for a, b in zip(sg, sr))
     gpu.close()
+
+
+def _run_with_env(monkeypatch, env, prm, state, nsteps, precision):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    gpu = H.HipStep(prm, precision=precision)
+    gpu.upload(state["r"], state["v"], state["rho"], state["itype"])
+    gpu.step(nsteps)
+    out, stats = gpu.download(), gpu.sort_stats()
+    gpu.close()
+    for k in env:
+        monkeypatch.delenv(k)
+    return out, stats
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_incremental_resort_equals_full_sort(monkeypatch, precision):
+    """The incremental re-sort (merge of the slots that kept their cell with
+    the few that changed) yields the permutation of the full stable radix
+    sort, so 200 dam-break steps are bit-identical either way; a re-sort
+    capacity of 4 forces the fallback (re-run with the full sort) on most
+    steps and changes nothing either."""
+    lay = G.dam_break()
+    prm = lay.params
+    print("incremental re-sort, dam break n", prm.ntotal, "precision", precision, flush=True)
+    full, sf = _run_with_env(monkeypatch, {"RRSPH_HIP_SORT": "full"}, prm, lay.state, 200, precision)
+    inc, si = _run_with_env(monkeypatch, {}, prm, lay.state, 200, precision)
+    assert sf["full"] == 200 and sf["incremental"] == 0
+    assert si["full"] == 1 and si["incremental"] == 199 and si["fallbacks"] == 0
+    for k in ("r", "v", "rho", "p", "itype"):
+        assert np.array_equal(full[k], inc[k]), k
+    tiny, st = _run_with_env(monkeypatch, {"RRSPH_HIP_RESORT_CAP": "4"}, prm, lay.state, 200, precision)
+    assert st["fallbacks"] > 0
+    for k in ("r", "v", "rho", "p", "itype"):
+        assert np.array_equal(full[k], tiny[k]), k
