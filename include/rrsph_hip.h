@@ -189,6 +189,43 @@ rrsph_status rrsph_hip_diagnostics(rrsph_ctx* ctx, rrsph_diagnostics* out);
 rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t ngauges, double search_n,
                               int32_t ptype, double* out);
 
+/* ---- differentiation ------------------------------------------------- */
+/* No reference counterpart: the reference has no derivative of its step.
+ * The adjoint differentiates the FP64 step (continuity density, cubic
+ * kernels, pressure, artificial viscosity, Lennard-Jones walls, XSPH,
+ * boundary motion with the Fourier piston, nwm = 6) with dt independent of
+ * the state (dt_correction_method 0, or 2 with dt_dynamic_mode 0). */
+#define RRSPH_MAX_GAUGES 64
+typedef struct rrsph_gauge_spec {
+    uint32_t n;                    /* gauges, at most RRSPH_MAX_GAUGES */
+    int32_t ptype;                 /* particle type the gauges look at (fluid: 2) */
+    double radius;                 /* window half-width R: weight (1 - (x - x_g)^2 / R^2)^3 */
+    double tau;                    /* softness; eta -> max y in the window as tau -> 0 */
+    double y_ref;                  /* shift of the exponent, keep (y - y_ref) / tau below ~700 */
+    double x[RRSPH_MAX_GAUGES];
+} rrsph_gauge_spec;
+
+/* Smooth surrogate of the gauge elevation on the current state:
+ * eta_g = y_ref + tau * log(sum_i w_i exp((y_i - y_ref) / tau) / sum_i w_i)
+ * over particles of type ptype (0 where the window is empty). */
+rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta);
+/* Replace the Fourier piston coefficients (nwm = 6):
+ * x(t) = sum_k a_k sin(k w t) + b_k cos(k w t), k = 1 .. nharm. */
+rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b);
+/* Differentiable run from the current state: nsteps steps, a checkpoint
+ * every checkpoint_every steps (device memory), the smooth gauges after
+ * every step into eta[step * spec->n + g] (host). */
+rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
+                                       const rrsph_gauge_spec* spec, double* eta);
+/* Reverse pass of the last adjoint_forward for eta_bar = dL/d eta (same
+ * layout): gradients with respect to the piston coefficients (nharm each)
+ * and, when the pointers are not NULL, with respect to the initial state
+ * (original particle order; r, v as x,y pairs).  Replays each checkpoint
+ * segment forward and runs the step adjoints backward; leaves the state as
+ * adjoint_forward left it. */
+rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
+                                        double* grad_r0, double* grad_v0, double* grad_rho0);
+
 /* ---- streams and timing ---------------------------------------------- */
 void* rrsph_hip_stream(rrsph_ctx* ctx);           /* hipStream_t the context launches on */
 rrsph_status rrsph_hip_set_graph(rrsph_ctx* ctx, int32_t enable);

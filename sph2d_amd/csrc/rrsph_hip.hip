@@ -18,6 +18,7 @@
 #include "diagnostics.hpp"
 #include "radix_sort.hpp"
 #include "resort.hpp"
+#include "adjoint.hpp"
 #include "step_kernels.hpp"
 #include "pair_tile.hpp"
 
@@ -76,6 +77,23 @@ double host_w(const rrsph_params_c& p, double dist, int skf) {
 
 }  // namespace
 
+// Checkpoints and per-step records of a differentiable run (FP64 only).
+struct AdjTape {
+    bool valid = false;
+    uint32_t nsteps = 0, every = 0;
+    GaugeSpec spec{};
+    size_t blob = 0;                     // bytes of one saved state
+    std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
+    void* seg = nullptr;                 // states of one replayed segment (every blobs)
+    std::vector<double> t, dt, dt_pred;  // per step: start time, whole_step dt, predict dt
+    std::vector<uint8_t> first, piston;
+    double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
+    double *eta_bar = nullptr, *partials = nullptr, *piston_sum = nullptr;
+    double *lam[2] = {nullptr, nullptr}, *direct = nullptr, *drho_adj = nullptr;
+    double4* out_adj = nullptr;
+    uint32_t* flag = nullptr;
+};
+
 struct rrsph_ctx {
     rrsph_params_c prm{};
     StepParams P{};
@@ -128,6 +146,7 @@ struct rrsph_ctx {
     bool incremental = true;            // false: every step runs the full radix sort
     bool need_full = true;              // skeyA does not describe A (after upload)
     uint64_t n_incremental = 0, n_full = 0, n_fallback = 0;
+    AdjTape adj;
 };
 
 namespace {
@@ -470,8 +489,20 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     return RRSPH_OK;
 }
 
+void free_tape(AdjTape& T) {
+    for (void* p : T.ck)
+        if (p) (void)hipFree(p);
+    T.ck.clear();
+    void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
+                    T.drho_adj, T.out_adj, T.flag};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    T = AdjTape{};
+}
+
 void free_all(rrsph_ctx* c) {
     drop_graph(c);
+    free_tape(c->adj);
     void* ptrs[] = {c->posA, c->posB, c->vaA, c->vaB, c->adA, c->pred, c->der, c->xs, c->pred_scratch, c->pres,
                     c->pres_scratch, c->oidA,
                     c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->digit_total,
@@ -946,6 +977,299 @@ rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, u
     if (full_steps) *full_steps = ctx->n_full;
     if (fallbacks) *fallbacks = ctx->n_fallback;
     if (last_changed) *last_changed = ctx->host_ctrl ? ctx->host_ctrl->resort_count : 0;
+    return RRSPH_OK;
+}
+
+// ---------------------------------------------------------------- differentiation
+static rrsph_status to_gauge_spec(rrsph_ctx* ctx, const rrsph_gauge_spec* in, GaugeSpec& G) {
+    if (!in || in->n > kMaxGauges || !(in->radius > 0) || !(in->tau > 0))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad gauge spec");
+    G = GaugeSpec{};
+    G.n = in->n;
+    G.ptype = in->ptype;
+    G.radius = in->radius;
+    G.tau = in->tau;
+    G.y_ref = in->y_ref;
+    for (uint32_t g = 0; g < in->n; ++g) G.x[g] = in->x[g];
+    return RRSPH_OK;
+}
+
+// smooth gauges of state A into eta/zs/ws (device, kMaxGauges entries each)
+static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec& G, double* partials, double* eta,
+                                          double* zs, double* ws) {
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    smooth_gauge_partials<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, G,
+                                                           partials);
+    smooth_gauge_final<<<1, kBlock, 0, ctx->stream>>>(nb, G, partials, eta, zs, ws);
+    HIP_TRY(hipGetLastError());
+    return RRSPH_OK;
+}
+
+static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
+    const StepParams& P = ctx->P;
+    if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint needs an FP64 context");
+    if (!ctx->fast_pair || P.density_treatment != 1)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
+                    "RRSPH_HIP: the adjoint needs continuity density, cubic kernels, approximation 2 and no "
+                    "artificial pressure");
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint needs dt independent of the state");
+    if (P.nwm == 4 || P.nwm == 5)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint supports the piston wave makers only");
+    return RRSPH_OK;
+}
+
+// one saved state: posA | vaA | adA | oidA | ctrl
+static rrsph_status save_state(rrsph_ctx* ctx, void* dst) {
+    const size_t n = ctx->P.n;
+    char* d = (char*)dst;
+    HIP_TRY(hipMemcpyAsync(d, ctx->posA, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + 16 * n, ctx->vaA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + 48 * n, ctx->adA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + 80 * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + 84 * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    return RRSPH_OK;
+}
+
+static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
+    const size_t n = ctx->P.n;
+    const char* d = (const char*)src;
+    HIP_TRY(hipMemcpyAsync(ctx->posA, d, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + 16 * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->adA, d + 48 * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + 80 * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + 84 * n, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    // the next step's cell keys from the restored positions, and a full sort
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    state_keys_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA, ctx->key_next);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->need_full = true;
+    return RRSPH_OK;
+}
+
+static rrsph_status record_step(rrsph_ctx* ctx, uint32_t n) {
+    const Ctrl& c = *ctx->host_ctrl;
+    AdjTape& T = ctx->adj;
+    if (c.halted) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
+    T.t[n] = c.cur_time;
+    T.dt[n] = c.dt_new;
+    T.dt_pred[n] = c.cur_dt;
+    T.first[n] = (uint8_t)c.cur_first;
+    T.piston[n] = (uint8_t)c.piston_active;
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
+    if (!ctx || !ctx->uploaded || !eta) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: smooth gauges need an FP64 context");
+    GaugeSpec G;
+    rrsph_status st = to_gauge_spec(ctx, spec, G);
+    if (st != RRSPH_OK) return st;
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    double* buf = nullptr;
+    HIP_TRY(hipMalloc((void**)&buf, (2 * kMaxGauges * (size_t)nb + 3 * kMaxGauges) * sizeof(double)));
+    double* out = buf + 2 * kMaxGauges * (size_t)nb;
+    st = enqueue_smooth_gauges(ctx, G, buf, out, out + kMaxGauges, out + 2 * kMaxGauges);
+    if (st == RRSPH_OK) {
+        HIP_TRY(hipMemcpyAsync(eta, out, G.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    HIP_TRY(hipFree(buf));
+    return st;
+}
+
+rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b) {
+    if (!ctx || nharm > RRSPH_MAX_HARMONICS || (nharm && (!a || !b)))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad piston coefficients");
+    ctx->prm.piston_omega = omega;
+    ctx->prm.piston_nharm = (int32_t)nharm;
+    ctx->P.piston_omega = omega;
+    ctx->P.piston_nharm = (int32_t)nharm;
+    for (uint32_t k = 0; k < RRSPH_MAX_HARMONICS; ++k) {
+        const double ak = k < nharm ? a[k] : 0.0, bk = k < nharm ? b[k] : 0.0;
+        ctx->prm.piston_a[k] = ak;
+        ctx->prm.piston_b[k] = bk;
+        ctx->P.piston_a[k] = ak;
+        ctx->P.piston_b[k] = bk;
+    }
+    drop_graph(ctx);  // the captured steps hold the old coefficients
+    ctx->adj.valid = false;
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
+                                       const rrsph_gauge_spec* spec, double* eta) {
+    if (!ctx || !ctx->uploaded || nsteps == 0 || checkpoint_every == 0 || !eta)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    rrsph_status st = adjoint_supported(ctx);
+    if (st != RRSPH_OK) return st;
+    GaugeSpec G;
+    st = to_gauge_spec(ctx, spec, G);
+    if (st != RRSPH_OK) return st;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    free_tape(ctx->adj);
+    AdjTape& T = ctx->adj;
+    const size_t n = ctx->P.n;
+    T.nsteps = nsteps;
+    T.every = std::min(checkpoint_every, nsteps);
+    T.spec = G;
+    T.blob = ((84 * n + sizeof(Ctrl)) + 255) / 256 * 256;
+    const uint32_t nck = (nsteps + T.every - 1) / T.every;
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    hipError_t e = hipSuccess;
+    T.ck.assign(nck + 1, nullptr);
+    for (auto& p : T.ck) p = dalloc<char>(T.blob, e);
+    T.seg = dalloc<char>(T.blob * T.every, e);
+    T.eta = dalloc<double>((size_t)nsteps * kMaxGauges, e);
+    T.zs = dalloc<double>((size_t)nsteps * kMaxGauges, e);
+    T.ws = dalloc<double>((size_t)nsteps * kMaxGauges, e);
+    T.eta_bar = dalloc<double>((size_t)nsteps * kMaxGauges, e);
+    T.partials = dalloc<double>(2 * kMaxGauges * (size_t)nb, e);
+    T.piston_sum = dalloc<double>(nsteps, e);
+    T.lam[0] = dalloc<double>(8 * n, e);
+    T.lam[1] = dalloc<double>(8 * n, e);
+    T.direct = dalloc<double>(5 * n, e);
+    T.drho_adj = dalloc<double>(n, e);
+    T.out_adj = dalloc<double4>(n, e);
+    T.flag = dalloc<uint32_t>(1, e);
+    if (e != hipSuccess) {
+        free_tape(ctx->adj);
+        return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
+    }
+    T.t.assign(nsteps, 0.0);
+    T.dt.assign(nsteps, 0.0);
+    T.dt_pred.assign(nsteps, 0.0);
+    T.first.assign(nsteps, 0);
+    T.piston.assign(nsteps, 0);
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        if (k % T.every == 0) {
+            st = save_state(ctx, T.ck[k / T.every]);
+            if (st != RRSPH_OK) return st;
+        }
+        st = run_steps(ctx, 1);
+        if (st != RRSPH_OK) return st;
+        st = record_step(ctx, k);
+        if (st != RRSPH_OK) return st;
+        st = enqueue_smooth_gauges(ctx, G, T.partials, T.eta + (size_t)k * kMaxGauges, T.zs + (size_t)k * kMaxGauges,
+                                   T.ws + (size_t)k * kMaxGauges);
+        if (st != RRSPH_OK) return st;
+    }
+    st = save_state(ctx, T.ck[nck]);
+    if (st != RRSPH_OK) return st;
+    std::vector<double> h((size_t)nsteps * kMaxGauges);
+    HIP_TRY(hipMemcpyAsync(h.data(), T.eta, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (uint32_t k = 0; k < nsteps; ++k)
+        for (uint32_t g = 0; g < G.n; ++g) eta[(size_t)k * G.n + g] = h[(size_t)k * kMaxGauges + g];
+    T.valid = true;
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
+                                        double* grad_r0, double* grad_v0, double* grad_rho0) {
+    if (!ctx || !eta_bar) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    AdjTape& T = ctx->adj;
+    if (!T.valid) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: run rrsph_hip_adjoint_forward first");
+    const StepParams& P = ctx->P;
+    const size_t n = P.n;
+    const uint32_t N = T.nsteps, G = T.spec.n;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    {
+        std::vector<double> h((size_t)N * kMaxGauges, 0.0);
+        for (uint32_t k = 0; k < N; ++k)
+            for (uint32_t g = 0; g < G; ++g) h[(size_t)k * kMaxGauges + g] = eta_bar[(size_t)k * G + g];
+        HIP_TRY(hipMemcpyAsync(T.eta_bar, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
+    HIP_TRY(hipMemsetAsync(T.piston_sum, 0, N * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(T.lam[0], 0, 8 * n * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(T.flag, 0, sizeof(uint32_t), ctx->stream));
+    const uint32_t nck = (N + T.every - 1) / T.every;
+    // the loss term of the final state
+    rrsph_status st = restore_state(ctx, T.ck[nck]);
+    if (st != RRSPH_OK) return st;
+    smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA, T.spec,
+                                                      T.eta_bar + (size_t)(N - 1) * kMaxGauges,
+                                                      T.zs + (size_t)(N - 1) * kMaxGauges,
+                                                      T.ws + (size_t)(N - 1) * kMaxGauges, T.lam[0]);
+    HIP_TRY(hipGetLastError());
+    int cur = 0;
+    Ctrl uc{};
+    for (int seg = (int)nck - 1; seg >= 0; --seg) {
+        const uint32_t n0 = (uint32_t)seg * T.every, n1 = std::min(n0 + T.every, N);
+        st = restore_state(ctx, T.ck[seg]);
+        if (st != RRSPH_OK) return st;
+        for (uint32_t k = n0; k < n1; ++k) {  // replay the segment, keeping the state before each step
+            st = save_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
+            if (st != RRSPH_OK) return st;
+            if (k + 1 < n1) {
+                st = run_steps(ctx, 1);
+                if (st != RRSPH_OK) return st;
+            }
+        }
+        for (int k = (int)n1 - 1; k >= (int)n0; --k) {
+            st = restore_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
+            if (st != RRSPH_OK) return st;
+            // sort and reorder state k exactly as its step did (predict with its dt)
+            uc = Ctrl{};
+            uc.cur_dt = T.dt_pred[k];
+            HIP_TRY(hipMemcpyAsync(ctx->util_ctrl, &uc, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+            enqueue_util_sort<double>(ctx);
+            StateB<double> B = stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch);
+            AdjStep S;
+            S.dt = T.dt[k];
+            S.dt_pred = T.dt_pred[k];
+            S.first = T.first[k];
+            S.piston = T.piston[k];
+            adj_prep<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.drho_adj, T.direct,
+                                                     T.piston_sum + k);
+            adj_pair<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
+                                                     T.direct, T.lam[1 - cur], T.flag);
+            if (k >= 1)  // loss term of state k (the gauges after step k - 1)
+                smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(
+                    P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA, T.spec,
+                    T.eta_bar + (size_t)(k - 1) * kMaxGauges, T.zs + (size_t)(k - 1) * kMaxGauges,
+                    T.ws + (size_t)(k - 1) * kMaxGauges, T.lam[1 - cur]);
+            HIP_TRY(hipGetLastError());
+            cur = 1 - cur;
+        }
+    }
+    // piston coefficients: dV/da_k = k w cos(k w t), dV/db_k = -k w sin(k w t)
+    std::vector<double> ps(N);
+    HIP_TRY(hipMemcpyAsync(ps.data(), T.piston_sum, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<double> lam(8 * n);
+    HIP_TRY(hipMemcpyAsync(lam.data(), T.lam[cur], 8 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t flag = 0;
+    HIP_TRY(hipMemcpyAsync(&flag, T.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const int nh = P.nwm == 6 ? P.piston_nharm : 0;
+    for (int k = 0; k < (int)RRSPH_MAX_HARMONICS && k < std::max(nh, 0); ++k) {
+        double ga = 0, gb = 0;
+        const double kw = (k + 1) * P.piston_omega;
+        for (uint32_t s = 0; s < N; ++s) {
+            ga += ps[s] * kw * std::cos(kw * T.t[s]);
+            gb -= ps[s] * kw * std::sin(kw * T.t[s]);
+        }
+        if (grad_a) grad_a[k] = ga;
+        if (grad_b) grad_b[k] = gb;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        if (grad_r0) {
+            grad_r0[2 * i] = lam[8 * i];
+            grad_r0[2 * i + 1] = lam[8 * i + 1];
+        }
+        if (grad_v0) {
+            grad_v0[2 * i] = lam[8 * i + 2];
+            grad_v0[2 * i + 1] = lam[8 * i + 3];
+        }
+        if (grad_rho0) grad_rho0[i] = lam[8 * i + 4];
+    }
+    // leave the context as the forward left it
+    st = restore_state(ctx, T.ck[nck]);
+    if (st != RRSPH_OK) return st;
+    if (flag) return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
     return RRSPH_OK;
 }
 

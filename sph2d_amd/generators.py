@@ -180,6 +180,25 @@ def piston_tank(delta: float = 0.0075, length: float = 20.0, depth: float = 0.5,
     return Layout(prm, r, v, rho, itype)
 
 
+def adjoint_tank(delta: float = 0.005, length: float = 8.0, depth: float = 0.4, wall_height: float = 1.0,
+                 nharm: int = 16, period: float = 1.6, sigma: float = 0.005, seed: int = 20240603,
+                 cfl: float = 0.2) -> Layout:
+    """Differentiable wave tank: the piston flume with a Fourier piston of
+    `nharm` harmonics at w = 2 pi / period, coefficients drawn from
+    N(0, sigma^2) with `seed`, and a fixed dt = cfl h / c (dt independent of
+    the state, as the adjoint requires)."""
+    rng = np.random.default_rng(seed)
+    a = tuple(rng.normal(0.0, sigma, nharm))
+    b = tuple(rng.normal(0.0, sigma, nharm))
+    lay = piston_tank(delta=delta, length=length, depth=depth, wall_height=wall_height,
+                      fourier=(a, b, 2 * math.pi / period))
+    prm = lay.params
+    prm.dt_correction_method = P.DT_CORRECTION_CONST_VALUE
+    prm.dt_dynamic_mode = 0
+    prm.dt = cfl * prm.hsml / prm.eos_c
+    return lay
+
+
 def c1_1m_delta() -> float:
     """Spacing that puts ~1M fluid particles in the 20 m x 0.5 m flume."""
     return 0.0031623

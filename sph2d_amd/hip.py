@@ -38,6 +38,7 @@ EXPORTED = [
     "rrsph_hip_fields", "rrsph_hip_cell_keys", "rrsph_hip_neighbours", "rrsph_hip_diagnostics", "rrsph_hip_gauges",
     "rrsph_hip_stream", "rrsph_hip_set_graph", "rrsph_hip_profile_steps", "rrsph_hip_kernel_class_count",
     "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats",
+    "rrsph_hip_smooth_gauges", "rrsph_hip_set_piston", "rrsph_hip_adjoint_forward", "rrsph_hip_adjoint_backward",
     "RRSPH_HIP_GetSpecificVersionMajor",
     "RRSPH_HIP_GetSpecificVersionMinor", "RRSPH_HIP_GetSpecificVersionPatch", "RRSPH_HIP_GetSpecificVersionName",
 ]
@@ -88,6 +89,27 @@ class Diagnostics(ctypes.Structure):
     ]
 
 
+MAX_GAUGES = 64
+
+
+class GaugeSpec(ctypes.Structure):
+    """rrsph_gauge_spec: smooth gauge surrogate (window radius, softness tau)."""
+    _fields_ = [("n", ctypes.c_uint32), ("ptype", ctypes.c_int32), ("radius", ctypes.c_double),
+                ("tau", ctypes.c_double), ("y_ref", ctypes.c_double), ("x", ctypes.c_double * MAX_GAUGES)]
+
+
+def gauge_spec(xs: Sequence[float], radius: float, tau: float, y_ref: float, ptype: int = 2) -> GaugeSpec:
+    if len(xs) > MAX_GAUGES:
+        raise ValueError(f"at most {MAX_GAUGES} gauges")
+    g = GaugeSpec()
+    g.n = len(xs)
+    g.ptype = int(ptype)
+    g.radius, g.tau, g.y_ref = float(radius), float(tau), float(y_ref)
+    for i, x in enumerate(xs):
+        g.x[i] = float(x)
+    return g
+
+
 class RrsphError(RuntimeError):
     def __init__(self, status: int, message: str):
         self.status = status
@@ -133,6 +155,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "rrsph_hip_time_steps": (st, [vp, ctypes.c_uint32, dp]),
         "rrsph_hip_sort_stats": (st, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64), up]),
+        "rrsph_hip_smooth_gauges": (st, [vp, ctypes.POINTER(GaugeSpec), dp]),
+        "rrsph_hip_set_piston": (st, [vp, ctypes.c_double, ctypes.c_uint32, dp, dp]),
+        "rrsph_hip_adjoint_forward": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(GaugeSpec), dp]),
+        "rrsph_hip_adjoint_backward": (st, [vp, dp, dp, dp, dp, dp, dp]),
         "RRSPH_HIP_GetSpecificVersionMajor": (ctypes.c_uint32, []),
         "RRSPH_HIP_GetSpecificVersionMinor": (ctypes.c_uint32, []),
         "RRSPH_HIP_GetSpecificVersionPatch": (ctypes.c_uint32, []),
@@ -272,6 +298,36 @@ class HipStep:
         ms = ctypes.c_double()
         self._check(self.L.rrsph_hip_time_steps(self.ctx, int(n), ctypes.byref(ms)))
         return ms.value
+
+    def smooth_gauges(self, spec: GaugeSpec) -> np.ndarray:
+        out = np.zeros(spec.n)
+        self._check(self.L.rrsph_hip_smooth_gauges(self.ctx, ctypes.byref(spec), _dp(out)))
+        return out
+
+    def set_piston(self, omega: float, a: Sequence[float], b: Sequence[float]):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if a.shape != b.shape:
+            raise ValueError("a and b need the same length")
+        self._check(self.L.rrsph_hip_set_piston(self.ctx, float(omega), len(a), _dp(a), _dp(b)))
+
+    def adjoint_forward(self, nsteps: int, every: int, spec: GaugeSpec) -> np.ndarray:
+        """Differentiable run from the uploaded state; gauges after each step,
+        shape (nsteps, ngauges)."""
+        eta = np.zeros((nsteps, spec.n))
+        self._check(self.L.rrsph_hip_adjoint_forward(self.ctx, int(nsteps), int(every), ctypes.byref(spec),
+                                                     _dp(eta)))
+        return eta
+
+    def adjoint_backward(self, eta_bar: np.ndarray, nharm: int) -> Dict[str, np.ndarray]:
+        """Gradients of the loss whose derivative with respect to the gauges
+        is eta_bar: piston coefficients a, b and the initial state."""
+        eta_bar = np.ascontiguousarray(eta_bar, dtype=np.float64)
+        ga, gb = np.zeros(max(nharm, 1)), np.zeros(max(nharm, 1))
+        gr, gv, grho = np.zeros((self.n, 2)), np.zeros((self.n, 2)), np.zeros(self.n)
+        self._check(self.L.rrsph_hip_adjoint_backward(self.ctx, _dp(eta_bar), _dp(ga), _dp(gb), _dp(gr), _dp(gv),
+                                                      _dp(grho)))
+        return {"a": ga[:nharm], "b": gb[:nharm], "r": gr, "v": gv, "rho": grho}
 
     def sort_stats(self) -> Dict[str, int]:
         """Steps sorted incrementally / fully, re-sort fallbacks, particles

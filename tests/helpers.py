@@ -93,3 +93,18 @@ def brute_force_sets(r, cut2, itype):
         m[j] = False
         out.append(np.nonzero(m)[0].astype(np.uint32))
     return out
+
+
+def smooth_gauges_np(r, itype, xs, radius, tau, y_ref, ptype=2):
+    """The smooth gauge surrogate in numpy (the definition in include/rrsph_hip.h):
+    eta = y_ref + tau log(sum w e / sum w), w = (1 - (x - x_g)^2 / R^2)^3, e = exp((y - y_ref) / tau)."""
+    sel = itype == ptype
+    x, y = r[sel, 0], r[sel, 1]
+    out = np.zeros(len(xs))
+    for g, xg in enumerate(xs):
+        u = 1.0 - (x - xg) ** 2 / radius ** 2
+        w = np.where(u > 0, u ** 3, 0.0)
+        ws = w.sum()
+        if ws > 0:
+            out[g] = y_ref + tau * np.log((w * np.exp((y - y_ref) / tau)).sum() / ws)
+    return out

@@ -45,8 +45,9 @@ def test_params_struct_layout_matches_c():
 #include <stddef.h>
 #include "rrsph_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu\n", sizeof(rrsph_params_c), offsetof(rrsph_params_c, piston_a),
-         offsetof(rrsph_params_c, precision), offsetof(rrsph_params_c, start_time), sizeof(rrsph_diagnostics));
+  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(rrsph_params_c), offsetof(rrsph_params_c, piston_a),
+         offsetof(rrsph_params_c, precision), offsetof(rrsph_params_c, start_time), sizeof(rrsph_diagnostics),
+         sizeof(rrsph_gauge_spec), offsetof(rrsph_gauge_spec, x));
   return 0;
 }
 """
@@ -56,7 +57,8 @@ int main(void) {
         open(c, "w").write(src)
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
-    size, off_a, off_prec, off_t, dsize = map(int, out)
+    size, off_a, off_prec, off_t, dsize, gsize, goff = map(int, out)
+    assert gsize == ctypes.sizeof(H.GaugeSpec) and goff == H.GaugeSpec.x.offset
     assert size == ctypes.sizeof(H.RrsphParams)
     assert off_a == H.RrsphParams.piston_a.offset
     assert off_prec == H.RrsphParams.precision.offset

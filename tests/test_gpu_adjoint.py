@@ -1,0 +1,192 @@
+"""The hand-written adjoint of the FP64 step (rrsph_hip_adjoint_*) against
+central finite differences of the GPU forward and of the FP64 oracle, and
+the smooth gauge surrogate against the max-in-window gauge."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import smooth_gauges_np
+from sph2d_amd import generators as G
+from sph2d_amd import hip as H
+from sph2d_amd.adjoint import DifferentiableRun, PistonGauges, central_difference
+
+pytestmark = pytest.mark.gpu
+
+DELTA = 0.02
+
+
+def _case(nsteps=60, every=16, nharm=4, mod=None):
+    lay = G.adjoint_tank(delta=DELTA, length=1.2, depth=0.3, wall_height=0.6, nharm=nharm, sigma=0.01, seed=5)
+    prm = lay.params
+    if mod:
+        mod(prm)
+    sim = H.HipStep(prm, precision=H.FP64)
+    spec = H.gauge_spec([0.08, 0.16, 0.3], radius=3 * DELTA, tau=DELTA / 4, y_ref=0.3)
+    run = DifferentiableRun(sim, dict(lay.state), nsteps, every, spec, prm.piston_omega)
+    return lay, run
+
+
+def _weights(shape, seed=11):
+    return np.random.default_rng(seed).normal(0.0, 1.0, shape)
+
+
+def test_smooth_gauge_tends_to_the_max_gauge():
+    """tau -> 0: the surrogate equals max y in the window (HeightTesting.h:25-54)
+    to within tau log(particles in the window)."""
+    lay = G.piston_tank(delta=DELTA, length=1.2, depth=0.3, wall_height=0.6)
+    sim = H.HipStep(lay.params, precision=H.FP64)
+    sim.upload(lay.r, lay.v, lay.rho, lay.itype)
+    sim.step(20)
+    xs = [0.2, 0.5, 0.9]
+    search_n = 3 * DELTA / lay.params.hsml
+    mx = sim.gauges(xs, search_n=search_n)
+    spec = H.gauge_spec(xs, radius=3 * DELTA, tau=1e-5, y_ref=float(np.max(mx)) + 1e-4)
+    sm = sim.smooth_gauges(spec)
+    assert np.all(sm <= mx + 1e-12) and np.all(mx - sm < 1e-4), (sm, mx)
+    st = sim.download()
+    spec2 = H.gauge_spec(xs, radius=3 * DELTA, tau=DELTA / 4, y_ref=0.3)
+    assert np.allclose(sim.smooth_gauges(spec2), smooth_gauges_np(st["r"], st["itype"], xs, 3 * DELTA, DELTA / 4, 0.3),
+                       rtol=0, atol=1e-13)
+    sim.close()
+
+
+def test_piston_gradient_matches_central_differences():
+    """dL/da_k, dL/db_k of L = sum c * eta over 60 steps (4 checkpoint
+    segments): adjoint against central differences of the GPU forward."""
+    lay, run = _case()
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    eta = run.forward(a, b)
+    c = _weights(eta.shape)
+    g = run.backward(c, len(a))
+    loss = lambda e: float(np.sum(c * e))
+    for which in ("a", "b"):
+        for k in range(len(a)):
+            fd = central_difference(run, a, b, loss, which, k, 1e-6)
+            ad = g[which][k]
+            assert abs(ad - fd) <= 1e-6 * max(abs(fd), 1e-3) + 1e-9, (which, k, ad, fd)
+    run.sim.close()
+
+
+def _moving_dot_case(mod=None, nsteps=24, every=5):
+    """A moving start (10 steps in) and a random direction d in positions,
+    fluid velocities and densities of every particle."""
+    lay, run = _case(nsteps=nsteps, every=every, mod=mod)
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    run.sim.upload(lay.r, lay.v, lay.rho, lay.itype)
+    run.sim.set_piston(prm.piston_omega, a, b)
+    run.sim.step(10)
+    st = run.sim.download()
+    run.state = {"r": st["r"], "v": st["v"], "rho": st["rho"], "itype": st["itype"]}
+    eta = run.forward(a, b)
+    c = _weights(eta.shape, seed=3)
+    g = run.backward(c, len(a))
+    rng = np.random.default_rng(7)
+    n = prm.ntotal
+    d = {"r": rng.normal(0, 1e-3 * DELTA, (n, 2)), "v": rng.normal(0, 1e-2, (n, 2)), "rho": rng.normal(0, 1e-1, n)}
+    d["v"][lay.itype < 0] = 0.0
+    pred = float(sum(np.sum(g[k] * d[k]) for k in d))
+    base = dict(run.state)
+    L0 = float(np.sum(c * eta))
+
+    def loss_at(eps):
+        run.state = {"r": base["r"] + eps * d["r"], "v": base["v"] + eps * d["v"], "rho": base["rho"] + eps * d["rho"],
+                     "itype": base["itype"]}
+        return float(np.sum(c * run.forward(a, b)))
+
+    return run, pred, L0, loss_at
+
+
+def test_initial_state_gradient_dot_test():
+    """<dL/dS0, d> equals the directional difference of L along d, with a
+    continuously differentiable artificial viscosity (alpha = 0, beta > 0:
+    the mu^2 term has no kink where v_ij . r_ij changes sign)."""
+    def smooth_av(p):
+        p.artificial_shear_visc = 0.0
+        p.artificial_bulk_visc = 0.5
+
+    run, pred, L0, loss_at = _moving_dot_case(smooth_av)
+    fd = (loss_at(1e-4) - loss_at(-1e-4)) / 2e-4
+    assert abs(pred - fd) <= 1e-6 * abs(fd), (pred, fd)
+    run.sim.close()
+
+
+def test_initial_state_gradient_is_a_subgradient_at_the_viscosity_switch():
+    """With alpha > 0 the viscosity term -alpha c mu switches off where
+    v_ij . r_ij = 0, a kink of the step: the one-sided differences along d
+    differ and stay apart as eps -> 0; the adjoint lies between them."""
+    run, pred, L0, loss_at = _moving_dot_case()
+    plus = [(loss_at(e) - L0) / e for e in (1e-4, 1e-5)]
+    minus = [(L0 - loss_at(-e)) / e for e in (1e-4, 1e-5)]
+    lo, hi = min(plus + minus), max(plus + minus)
+    assert lo - 1e-6 * abs(hi) <= pred <= hi + 1e-6 * abs(hi), (pred, plus, minus)
+    assert abs(pred - 0.5 * (plus[1] + minus[1])) <= 5e-3 * abs(pred)
+    run.sim.close()
+
+
+def test_torch_autograd_function():
+    """PistonGauges is a torch.autograd.Function whose backward is the adjoint."""
+    torch = pytest.importorskip("torch")
+    lay, run = _case(nsteps=20, every=6)
+    prm = lay.params
+    a = torch.tensor(prm.piston_a, dtype=torch.float64, requires_grad=True)
+    b = torch.tensor(prm.piston_b, dtype=torch.float64, requires_grad=True)
+    eta = PistonGauges.apply(a, b, run)
+    target = eta.detach() + 0.001
+    loss = ((eta - target) ** 2).sum()
+    loss.backward()
+    g = run.backward(2 * (eta.detach().numpy() - target.numpy()), len(prm.piston_a))
+    assert np.allclose(a.grad.numpy(), g["a"], rtol=1e-12, atol=0)
+    assert np.allclose(b.grad.numpy(), g["b"], rtol=1e-12, atol=0)
+    run.sim.close()
+
+
+def test_piston_gradient_matches_oracle_finite_differences():
+    """The adjoint gradient against central differences of the FP64 oracle
+    (its own forward and numpy gauges): agreement within 1e-3 relative."""
+    lay, run = _case(nsteps=40, every=10)
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    eta = run.forward(a, b)
+    c = _weights(eta.shape, seed=5)
+    g = run.backward(c, len(a))
+    spec = run.spec
+    xs = [spec.x[i] for i in range(spec.n)]
+
+    def oracle_loss(aa, bb):
+        p = copy.copy(prm)
+        p.piston_a, p.piston_b = tuple(aa), tuple(bb)
+        ora = oracle.Oracle(p)
+        ora.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=p.dt)
+        total = 0.0
+        for k in range(run.nsteps):
+            assert ora.step(1) == 0
+            st = ora.state()
+            total += float(np.sum(c[k] * smooth_gauges_np(st["r"], st["itype"], xs, spec.radius, spec.tau,
+                                                          spec.y_ref)))
+        return total
+
+    eps = 1e-6
+    for which, k in (("a", 0), ("b", 1)):
+        aa, bb = a.copy(), b.copy()
+        (aa if which == "a" else bb)[k] += eps
+        lp = oracle_loss(aa, bb)
+        (aa if which == "a" else bb)[k] -= 2 * eps
+        lm = oracle_loss(aa, bb)
+        fd = (lp - lm) / (2 * eps)
+        assert abs(g[which][k] - fd) <= 1e-3 * abs(fd), (which, k, g[which][k], fd)
+    run.sim.close()
+
+
+def test_adjoint_rejects_fp32_and_summation():
+    lay, _ = _case()
+    sim = H.HipStep(lay.params, precision=H.FP32)
+    sim.upload(lay.r, lay.v, lay.rho, lay.itype)
+    spec = H.gauge_spec([0.1], radius=0.06, tau=0.005, y_ref=0.3)
+    with pytest.raises(H.RrsphError) as e:
+        sim.adjoint_forward(5, 5, spec)
+    assert e.value.status == 7
+    sim.close()
