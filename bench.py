@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--adjoint-steps", type=int, default=2000, help="steps of the differentiable wave tank")
+    ap.add_argument("--adjoint-every", type=int, default=100, help="checkpoint interval of the adjoint")
+    ap.add_argument("--no-adjoint", action="store_true")
     return ap.parse_args()
 
 
@@ -79,6 +82,50 @@ def gpu_leg(layout, precision, steps, warmup, profile_steps):
         "dt": dt,
         "step": step,
         "diag": diag,
+    }
+
+
+def adjoint_leg(steps, every):
+    """The differentiable 2D wave tank (8 m x 1 m flume, depth 0.4 m,
+    delta 0.005 m, 16-harmonic Fourier piston, 4 gauges at x = 2..5 m),
+    FP64: plain forward steps, the forward with checkpoints and gauges, and
+    the reverse pass, each timed on the wall clock."""
+    import torch
+
+    from sph2d_amd import generators as G
+    from sph2d_amd import hip as H
+
+    lay = G.adjoint_tank()
+    prm = lay.params
+    g = H.HipStep(prm, precision=H.FP64)
+    delta = prm.delta
+    spec = H.gauge_spec([2.0, 3.0, 4.0, 5.0], radius=3 * delta, tau=delta / 4, y_ref=prm.depth)
+    # plain forward (graph replays)
+    g.upload(lay.r, lay.v, lay.rho, lay.itype)
+    g.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    # forward with checkpoints and gauges, then the reverse pass
+    g.upload(lay.r, lay.v, lay.rho, lay.itype)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eta = g.adjoint_forward(steps, every, spec)
+    t1 = time.perf_counter()
+    eta_bar = 2.0 * (eta - prm.depth)
+    grads = g.adjoint_backward(eta_bar, len(prm.piston_a))
+    t2 = time.perf_counter()
+    g.close()
+    return {
+        "workload": "2D differentiable wave tank 8 m x 1 m, depth 0.4 m, delta 0.005 m, %d particles (%d fluid), "
+                    "16-harmonic Fourier piston, gauges at x = 2, 3, 4, 5 m, %d steps, checkpoint every %d, FP64"
+                    % (prm.ntotal, prm.nfluid, steps, every),
+        "forward_s": round(fwd, 4),
+        "taped_forward_s": round(t1 - t0, 4),
+        "backward_s": round(t2 - t1, 4),
+        "grad_norm": float((grads["a"] ** 2).sum() + (grads["b"] ** 2).sum()) ** 0.5,
     }
 
 
@@ -144,6 +191,10 @@ def main():
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_leg(args.adjoint_steps, args.adjoint_every)
+
     cpu = None
     if not args.no_cpu:
         cpu_rate, cores, secs = cpu_leg(layout, args.cpu_steps)
@@ -184,7 +235,8 @@ def main():
             "per_kernel_ms": {k: round(v, 5) for k, v in r64["per_kernel_ms"].items()},
         },
         "fp32_over_fp64": None if r64 is None else round(r64["wall_s"] / r32["wall_s"], 3),
-        "adjoint_over_forward": None,
+        "adjoint_over_forward": None if adj is None else round(adj["backward_s"] / adj["forward_s"], 3),
+        "adjoint": adj,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

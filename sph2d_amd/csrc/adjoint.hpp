@@ -89,21 +89,30 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, cons
     }
 }
 
-// One block: sums the partials in block order (deterministic) and writes
-// eta[g], Z[g] = sum w e and Wsum[g] = sum w.
+// One block: wave w reduces gauges w, w + 4, ...: each lane sums a fixed
+// stride of the block partials, then a fixed shuffle tree (deterministic);
+// writes eta[g], Z[g] = sum w e and Wsum[g] = sum w.
 __global__ void __launch_bounds__(kBlock) smooth_gauge_final(uint32_t nblocks, GaugeSpec G,
                                                              const double* __restrict__ partials,
                                                              double* __restrict__ eta, double* __restrict__ zsum,
                                                              double* __restrict__ wsum) {
-    for (uint32_t g = threadIdx.x; g < G.n; g += kBlock) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
+    for (uint32_t g = wave; g < G.n; g += kBlock / kWave) {
         double z = 0, ww = 0;
-        for (uint32_t b = 0; b < nblocks; ++b) {
+        for (uint32_t b = lane; b < nblocks; b += kWave) {
             z += partials[(size_t)b * 2 * kMaxGauges + g];
             ww += partials[(size_t)b * 2 * kMaxGauges + kMaxGauges + g];
         }
-        eta[g] = ww > 0.0 ? G.y_ref + G.tau * log(z / ww) : 0.0;
-        zsum[g] = z;
-        wsum[g] = ww;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            z += __shfl_xor(z, off);
+            ww += __shfl_xor(ww, off);
+        }
+        if (lane == 0) {
+            eta[g] = ww > 0.0 ? G.y_ref + G.tau * log(z / ww) : 0.0;
+            zsum[g] = z;
+            wsum[g] = ww;
+        }
     }
 }
 
@@ -150,11 +159,35 @@ __global__ void __launch_bounds__(kBlock) state_keys_kernel(StepParams P, const 
 
 // ------------------------------------------------------------------ step adjoint
 struct AdjStep {
+    double t;         // time at the start of the step
     double dt;        // dt of whole_step and of the time advance
     double dt_pred;   // dt of predict_half_step
     uint32_t first;   // first step: v and rho advance with dt / 2
     uint32_t piston;  // the wave maker moves its particles this step
+    uint32_t halted;  // the step did not run
+    uint32_t pad;
 };
+
+// After a step: what its adjoint needs from the control block.
+__global__ void adj_record(const Ctrl* __restrict__ ctrl, AdjStep* __restrict__ rec) {
+    const Ctrl& c = *ctrl;
+    AdjStep r;
+    r.t = c.cur_time;
+    r.dt = c.dt_new;
+    r.dt_pred = c.cur_dt;
+    r.first = c.cur_first;
+    r.piston = c.piston_active;
+    r.halted = c.halted;
+    r.pad = 0;
+    *rec = r;
+}
+
+// The control block of a utility sort that predicts with dt_pred.
+__global__ void adj_util_ctrl(Ctrl* __restrict__ uc, const AdjStep* __restrict__ rec) {
+    Ctrl c{};
+    c.cur_dt = rec->dt_pred;
+    *uc = c;
+}
 
 // whole_step reverse.  out_adj[s] = (a_x, a_y, av_x, av_y) adjoints,
 // drho_adj[s]; direct[s] = (r_x, r_y, v_x, v_y, rho) adjoints that do not

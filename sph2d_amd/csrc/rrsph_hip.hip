@@ -85,8 +85,8 @@ struct AdjTape {
     size_t blob = 0;                     // bytes of one saved state
     std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
     void* seg = nullptr;                 // states of one replayed segment (every blobs)
-    std::vector<double> t, dt, dt_pred;  // per step: start time, whole_step dt, predict dt
-    std::vector<uint8_t> first, piston;
+    AdjStep* rec = nullptr;              // per step (device)
+    std::vector<AdjStep> hrec;           // per step (host copy)
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
     double *eta_bar = nullptr, *partials = nullptr, *piston_sum = nullptr;
     double *lam[2] = {nullptr, nullptr}, *direct = nullptr, *drho_adj = nullptr;
@@ -494,7 +494,7 @@ void free_tape(AdjTape& T) {
         if (p) (void)hipFree(p);
     T.ck.clear();
     void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
-                    T.drho_adj, T.out_adj, T.flag};
+                    T.drho_adj, T.out_adj, T.flag, T.rec};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     T = AdjTape{};
@@ -1039,26 +1039,66 @@ static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
     HIP_TRY(hipMemcpyAsync(ctx->adA, d + 48 * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->oidA, d + 80 * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + 84 * n, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
     // the next step's cell keys from the restored positions, and a full sort
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     state_keys_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA, ctx->key_next);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->need_full = true;
     return RRSPH_OK;
 }
 
-static rrsph_status record_step(rrsph_ctx* ctx, uint32_t n) {
-    const Ctrl& c = *ctx->host_ctrl;
-    AdjTape& T = ctx->adj;
-    if (c.halted) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
-    T.t[n] = c.cur_time;
-    T.dt[n] = c.dt_new;
-    T.dt_pred[n] = c.cur_dt;
-    T.first[n] = (uint8_t)c.cur_first;
-    T.piston[n] = (uint8_t)c.piston_active;
+// restore_state, then bring the host mirror of the control block up to date
+static rrsph_status restore_state_sync(rrsph_ctx* ctx, const void* src) {
+    rrsph_status st = restore_state(ctx, src);
+    if (st != RRSPH_OK) return st;
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return RRSPH_OK;
+}
+
+// Launch steps [k0, k1) of the tape without host synchronisation: every
+// step's record, the checkpoints (forward) or segment states (replay), and
+// the gauges (forward).  Returns the status read after the last step.
+static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base) {
+    AdjTape& T = ctx->adj;
+    for (uint32_t k = k0; k < k1; ++k) {
+        rrsph_status st;
+        if (forward && k % T.every == 0) {
+            st = save_state(ctx, T.ck[k / T.every]);
+            if (st != RRSPH_OK) return st;
+        }
+        if (!forward) {
+            st = save_state(ctx, (char*)seg_base + (size_t)(k - k0) * T.blob);
+            if (st != RRSPH_OK) return st;
+            if (k + 1 == k1) break;  // the state before the segment's last step is all the adjoint needs
+        }
+        st = launch_steps(ctx, 1);
+        if (st != RRSPH_OK) return st;
+        if (forward) {
+            adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec + k);
+            st = enqueue_smooth_gauges(ctx, T.spec, T.partials, T.eta + (size_t)k * kMaxGauges,
+                                       T.zs + (size_t)k * kMaxGauges, T.ws + (size_t)k * kMaxGauges);
+            if (st != RRSPH_OK) return st;
+        }
+    }
+    return sync_status(ctx);
+}
+
+// adjoint_run with the incremental re-sort; if a step needed the full-sort
+// fallback (a host round trip), redo the range with the full sort only.
+static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base,
+                                        const void* start) {
+    rrsph_status st = adjoint_run(ctx, k0, k1, forward, seg_base);
+    if (st != RRSPH_OK) return st;
+    if (!(ctx->host_ctrl->halted && ctx->host_ctrl->status == kStatusResort)) return RRSPH_OK;
+    ++ctx->n_fallback;
+    st = restore_state_sync(ctx, start);
+    if (st != RRSPH_OK) return st;
+    const bool inc = ctx->incremental;
+    ctx->incremental = false;
+    st = adjoint_run(ctx, k0, k1, forward, seg_base);
+    ctx->incremental = inc;
+    return st;
 }
 
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
@@ -1138,28 +1178,21 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
         free_tape(ctx->adj);
         return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
     }
-    T.t.assign(nsteps, 0.0);
-    T.dt.assign(nsteps, 0.0);
-    T.dt_pred.assign(nsteps, 0.0);
-    T.first.assign(nsteps, 0);
-    T.piston.assign(nsteps, 0);
-    for (uint32_t k = 0; k < nsteps; ++k) {
-        if (k % T.every == 0) {
-            st = save_state(ctx, T.ck[k / T.every]);
-            if (st != RRSPH_OK) return st;
-        }
-        st = run_steps(ctx, 1);
-        if (st != RRSPH_OK) return st;
-        st = record_step(ctx, k);
-        if (st != RRSPH_OK) return st;
-        st = enqueue_smooth_gauges(ctx, G, T.partials, T.eta + (size_t)k * kMaxGauges, T.zs + (size_t)k * kMaxGauges,
-                                   T.ws + (size_t)k * kMaxGauges);
-        if (st != RRSPH_OK) return st;
+    T.rec = dalloc<AdjStep>(nsteps, e);
+    if (e != hipSuccess) {
+        free_tape(ctx->adj);
+        return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
     }
+    st = adjoint_run_checked(ctx, 0, nsteps, true, nullptr, T.ck[0]);
+    if (st != RRSPH_OK) return st;
+    if (ctx->host_ctrl->halted)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
     st = save_state(ctx, T.ck[nck]);
     if (st != RRSPH_OK) return st;
     std::vector<double> h((size_t)nsteps * kMaxGauges);
     HIP_TRY(hipMemcpyAsync(h.data(), T.eta, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    T.hrec.resize(nsteps);
+    HIP_TRY(hipMemcpyAsync(T.hrec.data(), T.rec, nsteps * sizeof(AdjStep), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < nsteps; ++k)
         for (uint32_t g = 0; g < G.n; ++g) eta[(size_t)k * G.n + g] = h[(size_t)k * kMaxGauges + g];
@@ -1196,33 +1229,21 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
                                                       T.ws + (size_t)(N - 1) * kMaxGauges, T.lam[0]);
     HIP_TRY(hipGetLastError());
     int cur = 0;
-    Ctrl uc{};
     for (int seg = (int)nck - 1; seg >= 0; --seg) {
         const uint32_t n0 = (uint32_t)seg * T.every, n1 = std::min(n0 + T.every, N);
+        // replay the segment from its checkpoint, keeping the state before each step
         st = restore_state(ctx, T.ck[seg]);
         if (st != RRSPH_OK) return st;
-        for (uint32_t k = n0; k < n1; ++k) {  // replay the segment, keeping the state before each step
-            st = save_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
-            if (st != RRSPH_OK) return st;
-            if (k + 1 < n1) {
-                st = run_steps(ctx, 1);
-                if (st != RRSPH_OK) return st;
-            }
-        }
+        st = adjoint_run_checked(ctx, n0, n1, false, T.seg, T.ck[seg]);
+        if (st != RRSPH_OK) return st;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
             st = restore_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
             if (st != RRSPH_OK) return st;
             // sort and reorder state k exactly as its step did (predict with its dt)
-            uc = Ctrl{};
-            uc.cur_dt = T.dt_pred[k];
-            HIP_TRY(hipMemcpyAsync(ctx->util_ctrl, &uc, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+            adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
             enqueue_util_sort<double>(ctx);
             StateB<double> B = stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch);
-            AdjStep S;
-            S.dt = T.dt[k];
-            S.dt_pred = T.dt_pred[k];
-            S.first = T.first[k];
-            S.piston = T.piston[k];
+            const AdjStep& S = T.hrec[k];
             adj_prep<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.drho_adj, T.direct,
                                                      T.piston_sum + k);
             adj_pair<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
@@ -1249,8 +1270,8 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         double ga = 0, gb = 0;
         const double kw = (k + 1) * P.piston_omega;
         for (uint32_t s = 0; s < N; ++s) {
-            ga += ps[s] * kw * std::cos(kw * T.t[s]);
-            gb -= ps[s] * kw * std::sin(kw * T.t[s]);
+            ga += ps[s] * kw * std::cos(kw * T.hrec[s].t);
+            gb -= ps[s] * kw * std::sin(kw * T.hrec[s].t);
         }
         if (grad_a) grad_a[k] = ga;
         if (grad_b) grad_b[k] = gb;
@@ -1267,7 +1288,7 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         if (grad_rho0) grad_rho0[i] = lam[8 * i + 4];
     }
     // leave the context as the forward left it
-    st = restore_state(ctx, T.ck[nck]);
+    st = restore_state_sync(ctx, T.ck[nck]);
     if (st != RRSPH_OK) return st;
     if (flag) return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
     return RRSPH_OK;
