@@ -15,8 +15,8 @@
 //   Output      : setup_output(load_r, load_itype, load_v, load_p, load_rho),
 //                 start_step(t), update_step(t, step), finish_step()
 //                 (sources/RRSPH_COMMON/include/Output.h:11-31)
-//   FloatnArray : vheap_darray_floatn -- get_flt2() returning an indexable
-//                 array of {x, y}; constructible from a size
+//   FloatnArray : vheap_darray_floatn -- get_flt2() / get_flt3() returning an
+//                 indexable array of {x, y} / {x, y, z}; constructible from a size
 //   FloatArray / IntArray : heap_darray<rr_float> / heap_darray<rr_int>
 // Errors are thrown as std::runtime_error with the reference's messages.
 #pragma once
@@ -166,6 +166,41 @@ private:
     uint32_t n_ = 0;
 };
 
+// Copy between the reference's vector arrays and AoS doubles (dim per particle).
+template <class FloatnArray> void floatn_to_aos(FloatnArray& a, size_t n, int dim, std::vector<double>& out) {
+    out.resize(dim * n);
+    if (dim == 3) {
+        auto& f = a.get_flt3();
+        for (size_t i = 0; i < n; ++i) {
+            out[3 * i] = f(i).x;
+            out[3 * i + 1] = f(i).y;
+            out[3 * i + 2] = f(i).z;
+        }
+    } else {
+        auto& f = a.get_flt2();
+        for (size_t i = 0; i < n; ++i) {
+            out[2 * i] = f(i).x;
+            out[2 * i + 1] = f(i).y;
+        }
+    }
+}
+template <class FloatnArray> void aos_to_floatn(const std::vector<double>& in, size_t n, int dim, FloatnArray& a) {
+    if (dim == 3) {
+        auto& f = a.get_flt3();
+        for (size_t i = 0; i < n; ++i) {
+            f(i).x = in[3 * i];
+            f(i).y = in[3 * i + 1];
+            f(i).z = in[3 * i + 2];
+        }
+    } else {
+        auto& f = a.get_flt2();
+        for (size_t i = 0; i < n; ++i) {
+            f(i).x = in[2 * i];
+            f(i).y = in[2 * i + 1];
+        }
+    }
+}
+
 // Drop-in for time_integration(r_var, v_var, rho, p, itype): runs the
 // reference's loop `while (time <= simulation_time)` on the GPU, keeps
 // params.dt current (the reference mutates it inside the step) and serves
@@ -173,36 +208,26 @@ private:
 template <class Params, class Output, class FloatnArray, class FloatArray, class IntArray>
 void hip_time_integration(Params& params, Output& output, FloatnArray& r_var, FloatnArray& v_var, FloatArray& rho,
                           FloatArray& p, IntArray& itype, const Options& opt = Options()) {
-    if (params.dim != 2) throw Error(RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: only dim = 2 is implemented");
     const size_t n = params.ntotal;
-    std::vector<double> hr(2 * n), hv(2 * n), hrho(n);
+    const int dim = (int)params.dim;
+    std::vector<double> hr, hv, hrho(n);
     std::vector<int32_t> hit(n);
-    {
-        auto& r = r_var.get_flt2();
-        auto& v = v_var.get_flt2();
-        for (size_t i = 0; i < n; ++i) {
-            hr[2 * i] = r(i).x;
-            hr[2 * i + 1] = r(i).y;
-            hv[2 * i] = v(i).x;
-            hv[2 * i + 1] = v(i).y;
-            hrho[i] = rho(i);
-            hit[i] = (int32_t)itype(i);
-        }
-    }
     Context ctx(make_params(params, opt));
+    floatn_to_aos(r_var, n, dim, hr);
+    floatn_to_aos(v_var, n, dim, hv);
+    for (size_t i = 0; i < n; ++i) {
+        hrho[i] = rho(i);
+        hit[i] = (int32_t)itype(i);
+    }
     ctx.upload(hr.data(), hv.data(), hrho.data(), hit.data());
 
     const size_t maxn = params.maxn;
-    auto load_floatn = [&ctx, n, maxn](bool vel) {
-        std::vector<double> buf(2 * n);
+    auto load_floatn = [&ctx, n, maxn, dim](bool vel) {
+        std::vector<double> buf(dim * n);
         if (vel) ctx.download(RRSPH_FIELD_V, nullptr, buf.data(), nullptr, nullptr, nullptr);
         else ctx.download(RRSPH_FIELD_R, buf.data(), nullptr, nullptr, nullptr, nullptr);
         auto out = std::make_shared<FloatnArray>(maxn);
-        auto& a = out->get_flt2();
-        for (size_t i = 0; i < n; ++i) {
-            a(i).x = buf[2 * i];
-            a(i).y = buf[2 * i + 1];
-        }
+        aos_to_floatn(buf, n, dim, *out);
         return out;
     };
     auto load_scalar = [&ctx, n, maxn](uint32_t field) {
@@ -238,13 +263,9 @@ void hip_time_integration(Params& params, Output& output, FloatnArray& r_var, Fl
     std::vector<double> hp(n);
     ctx.download(RRSPH_FIELD_R | RRSPH_FIELD_V | RRSPH_FIELD_RHO | RRSPH_FIELD_P | RRSPH_FIELD_ITYPE, hr.data(),
                  hv.data(), hrho.data(), hp.data(), hit.data());
-    auto& r = r_var.get_flt2();
-    auto& v = v_var.get_flt2();
+    aos_to_floatn(hr, n, dim, r_var);
+    aos_to_floatn(hv, n, dim, v_var);
     for (size_t i = 0; i < n; ++i) {
-        r(i).x = hr[2 * i];
-        r(i).y = hr[2 * i + 1];
-        v(i).x = hv[2 * i];
-        v(i).y = hv[2 * i + 1];
         rho(i) = hrho[i];
         p(i) = hp[i];
         itype(i) = hit[i];

@@ -68,10 +68,11 @@ struct StepParams {
     uint32_t n;
     uint32_t nfluid;
     uint32_t max_nbr;
-    int32_t ncx, ncy;
-    uint32_t ncells;        // = ncx * ncy; also the dead-particle key
+    int32_t dim;            // 2 or 3
+    int32_t ncx, ncy, ncz;  // ncz = 1 in 2D
+    uint32_t ncells;        // = ncx * ncy (* ncz); also the dead-particle key
     // grid
-    double xmin, ymin, xmax, ymax, cell;
+    double xmin, ymin, xmax, ymax, zmin, zmax, cell;
     double cut2;            // (k h)^2, compared in FP64
     // physics
     double h, mass, rho0, g;
@@ -100,7 +101,7 @@ struct StepParams {
     // wave maker
     int32_t boundary_motion;
     int32_t nwm;
-    int32_t nwm_axis;       // 0 x, 1 y, -1 none
+    int32_t nwm_axis;       // 0 x, 1 y, 2 z, -1 none
     uint32_t nwm_start, nwm_end;
     double nwm_time_start;
     double nwm_freq, nwm_delta, nwm_S0, nwm_e2;

@@ -8,6 +8,7 @@
 
 #include "device_common.hpp"
 #include "step_kernels.hpp"
+#include "step3d.hpp"
 
 namespace rrsph {
 
@@ -54,6 +55,33 @@ template <class R> __global__ void __launch_bounds__(kBlock) diag_partials(StepP
         if ((int)va.w > 0) {
             const double vx = va.x, vy = va.y, rho = va.z;
             const double v2 = vx * vx + vy * vy;
+            acc[0] += 1.0;
+            acc[2] += 0.5 * P.mass * v2;
+            acc[3] += P.mass * P.g * p.y;
+            acc[4] += P.mass * tait_energy(rho, P.rho0, P.eos_B);
+            acc[5] = fmax(acc[5], sqrt(v2));
+            acc[6] = fmax(acc[6], -rho);
+            acc[7] = fmax(acc[7], rho);
+        }
+    }
+    for (int k = 0; k < kDiagSlots; ++k) {
+        const double r = k < 5 ? block_reduce_sum(acc[k], sh) : block_reduce_max(acc[k], sh);
+        if (threadIdx.x == 0) red[k * kDiagMaxBlocks + blockIdx.x] = r;
+    }
+}
+
+template <class R> __global__ void __launch_bounds__(kBlock) diag_partials3(StepParams P, StateA3<R> A, double* red) {
+    __shared__ double sh[kBlock / kWave];
+    double acc[kDiagSlots] = {0, 0, 0, 0, 0, 0, -INFINITY, -INFINITY};
+    for (uint32_t s = blockIdx.x * kBlock + threadIdx.x; s < P.n; s += gridDim.x * kBlock) {
+        const double4 p = A.pos[s];
+        const real4<R> va = A.va[s];
+        const bool inside =
+            p.x < P.xmax && p.x > P.xmin && p.y < P.ymax && p.y > P.ymin && p.z < P.zmax && p.z > P.zmin;
+        acc[1] += inside ? 0.0 : 1.0;
+        if ((int)p.w > 0) {
+            const double vx = va.x, vy = va.y, vz = va.z, rho = va.w;
+            const double v2 = vx * vx + vy * vy + vz * vz;
             acc[0] += 1.0;
             acc[2] += 0.5 * P.mass * v2;
             acc[3] += P.mass * P.g * p.y;

@@ -20,6 +20,7 @@
 #include "resort.hpp"
 #include "adjoint.hpp"
 #include "step_kernels.hpp"
+#include "step3d.hpp"
 #include "pair_tile.hpp"
 
 using namespace rrsph;
@@ -113,6 +114,7 @@ struct rrsph_ctx {
     double2 *posA = nullptr, *posB = nullptr;
     void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
          *pred_scratch = nullptr, *pres = nullptr, *pres_scratch = nullptr;
+    void *pw = nullptr, *pw_scratch = nullptr;  // 3D: p / rho^2 (2D keeps it in pred.w)
     bool fast_pair = false;
     int pair_impl = 1;  // 0 general, 1 two-phase walk with LDS hit lists, 2 LDS tiles, 3 single-phase walk
     uint32_t *oidA = nullptr, *oidB = nullptr;
@@ -168,6 +170,13 @@ template <class R> StateA<R> stateA(rrsph_ctx* c) {
 }
 template <class R> StateB<R> stateB(rrsph_ctx* c, void* pred, void* pres) {
     return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB, (R*)pres};
+}
+// 3D: the position buffers hold double4 (two double2 slots per particle)
+template <class R> StateA3<R> stateA3(rrsph_ctx* c) {
+    return StateA3<R>{(double4*)c->posA, (real4<R>*)c->vaA, (real4<R>*)c->adA, c->oidA};
+}
+template <class R> StateB3<R> stateB3(rrsph_ctx* c, void* pred, void* pw, void* pres) {
+    return StateB3<R>{(double4*)c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, (R*)pw, (R*)pres, c->oidB};
 }
 
 uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
@@ -288,7 +297,51 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof, bool full_s
     }
 }
 
+template <class R> void enqueue_step3_t(rrsph_ctx* c, Profiler* prof, bool full_sort) {
+    Launcher L{c, prof};
+    const StepParams& P = c->P;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
+    L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+    if (full_sort) enqueue_sort(c, c->ctrl, L);
+    else enqueue_resort(c, L);
+    StateA3<R> A = stateA3<R>(c);
+    StateB3<R> B = stateB3<R>(c, c->pred, c->pw, c->pres);
+    L(K_REORDER, [&] {
+        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
+                                                         c->maxslots, c->skeyA);
+    });
+    if (P.density_treatment == 0) {
+        L(K_DENSITY_SUM, [&] {
+            density_sum3_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB);
+        });
+    }
+    real4<R>* der = (real4<R>*)c->der;
+    real4<R>* xs = (real4<R>*)c->xs;
+    auto pair = [&](auto kernel) {
+        L(K_PAIR, [&] {
+            kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, B, c->sorted_keys, c->LB, A, c->key_next, der, xs,
+                                                     c->maxslots);
+        });
+    };
+    if (c->fused) {
+        if (c->export_fields) pair(pair3_kernel<R, true, true>);
+        else pair(pair3_kernel<R, true, false>);
+    } else {
+        pair(pair3_kernel<R, false, true>);
+        L(K_INTEGRATE, [&] {
+            integrate3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, B, der, xs, A, c->key_next, c->maxslots,
+                                                               c->sorted_keys);
+        });
+    }
+}
+
 void enqueue_step(rrsph_ctx* c, Profiler* prof, bool full_sort) {
+    if (c->P.dim == 3) {
+        if (c->fp32) enqueue_step3_t<float>(c, prof, full_sort);
+        else enqueue_step3_t<double>(c, prof, full_sort);
+        return;
+    }
     if (c->fp32) enqueue_step_t<float>(c, prof, full_sort);
     else enqueue_step_t<double>(c, prof, full_sort);
 }
@@ -324,6 +377,13 @@ template <class R> void enqueue_util_sort(rrsph_ctx* c) {
     Launcher L{c, nullptr};
     enqueue_sort(c, c->util_ctrl, L);
     const uint32_t nb = grid_blocks(c->P.n, kBlock);
+    if (c->P.dim == 3) {
+        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm,
+                                                         stateA3<R>(c),
+                                                         stateB3<R>(c, c->pred_scratch, c->pw_scratch, c->pres_scratch),
+                                                         c->LB, c->maxslots + 2 * kMaxSlots, nullptr);
+        return;
+    }
     StateA<R> A = stateA<R>(c);
     StateB<R> B = stateB<R>(c, c->pred_scratch, c->pres_scratch);
     reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
@@ -359,36 +419,45 @@ rrsph_status sync_status(rrsph_ctx* ctx) {
 rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     StepParams& P = ctx->P;
     P = StepParams{};
-    if (p.dim != 2) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: only dim = 2 is implemented");
+    if (p.dim != 2 && p.dim != 3) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: dim must be 2 or 3");
+    const bool d3 = p.dim == 3;
     if (p.ntotal == 0 || p.max_neighbours < 2 || !(p.hsml > 0) || !(p.cell_scale_k > 0))
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: invalid sizes");
     if (p.nfluid > p.ntotal) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: nfluid > ntotal");
     P.n = p.ntotal;
     P.nfluid = p.nfluid;
     P.max_nbr = p.max_neighbours;
+    P.dim = p.dim;
     P.cell = p.cell_scale_k * p.hsml;
+    // Input.cpp:45-70: at most 2^16 cells per axis in 2D, 2^10 in 3D
     const double tx = (p.x_maxgeom - p.x_mingeom) / P.cell, ty = (p.y_maxgeom - p.y_mingeom) / P.cell;
-    if (!(tx >= 0) || !(ty >= 0) || tx > 65536.0 || ty > 65536.0)
+    const double tz = d3 ? (p.z_maxgeom - p.z_mingeom) / P.cell : 0.0;
+    const double max_id = d3 ? 1024.0 : 65536.0;
+    if (!(tx >= 0) || !(ty >= 0) || !(tz >= 0) || tx > max_id || ty > max_id || (d3 && tz >= max_id))
         return fail(ctx, RRSPH_ERR_TOO_MANY_CELLS, "can't make grid with so many cells");
     P.ncx = (int32_t)(uint64_t)tx + 1;
     P.ncy = (int32_t)(uint64_t)ty + 1;
-    const uint64_t ncells = (uint64_t)P.ncx * (uint64_t)P.ncy;
+    P.ncz = d3 ? (int32_t)(uint64_t)tz + 1 : 1;
+    const uint64_t ncells = (uint64_t)P.ncx * (uint64_t)P.ncy * (uint64_t)P.ncz;
     if (ncells + 2 >= (1ull << 31)) return fail(ctx, RRSPH_ERR_TOO_MANY_CELLS, "can't make grid with so many cells");
     P.ncells = (uint32_t)ncells;
     P.xmin = p.x_mingeom;
     P.ymin = p.y_mingeom;
+    P.zmin = p.z_mingeom;
     P.xmax = p.x_maxgeom;
     P.ymax = p.y_maxgeom;
+    P.zmax = p.z_maxgeom;
     P.cut2 = P.cell * P.cell;
     P.h = p.hsml;
     P.mass = p.mass;
     P.rho0 = p.rho0;
     P.g = p.g;
     const double h = p.hsml, pi = p.pi;
-    P.f_cubic = double(15) / (double(7) * pi * (h * h));
-    P.f_gauss = 1 / (pi * (h * h));
-    P.f_wend = 7 / (4 * pi * (h * h));
-    P.f_desb = 5 / (16 * pi * (h * h));
+    // SmoothingKernel.h:22-23, 93-94, 136-137, 194-195
+    P.f_cubic = d3 ? double(3) / (double(2) * pi * (h * h * h)) : double(15) / (double(7) * pi * (h * h));
+    P.f_gauss = d3 ? 1 / (pi * std::sqrt(pi) * (h * h * h)) : 1 / (pi * (h * h));
+    P.f_wend = d3 ? 21 / (16 * pi * (h * h * h)) : 7 / (4 * pi * (h * h));
+    P.f_desb = d3 ? 15 / (pi * (4 * h) * (4 * h) * (4 * h)) : 5 / (16 * pi * (h * h));
     P.skf_density = p.density_skf;
     P.skf_intf = p.intf_skf;
     P.skf_av = p.artificial_viscosity_skf;
@@ -427,6 +496,9 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     if (p.nwm == 2 || p.nwm == 3 || p.nwm == 6) {
         if (p.nwm_direction == 1) P.nwm_axis = 0;
         else if (p.nwm_direction == 2) P.nwm_axis = 1;
+        else if (p.nwm_direction == 3 && d3) P.nwm_axis = 2;
+        else if (p.nwm_direction == 4 && d3 && p.boundary_motion)
+            return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: radial wave makers are not implemented");
     } else if (p.nwm == 5) {
         P.nwm_axis = 0;
     }
@@ -486,6 +558,10 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&
                      p.average_velocity_skf == 1 && p.intf_sph_approximation == 2 && !p.artificial_pressure &&
                      (p.density_treatment == 0 || p.density_treatment == 1) && std::fabs(p.cell_scale_k - 2.0) == 0.0;
+    if (d3 && !ctx->fast_pair)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
+                    "RRSPH_HIP: 3D supports cubic kernels for every term, approximation 2, no artificial pressure, "
+                    "summation or continuity density");
     return RRSPH_OK;
 }
 
@@ -508,7 +584,8 @@ void free_all(rrsph_ctx* c) {
                     c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->digit_total,
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
-                    c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg_before, c->rtile, c->chg};
+                    c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg_before, c->rtile, c->chg, c->pw,
+                    c->pw_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -542,8 +619,9 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     const size_t n = ctx->P.n;
     const size_t r4 = ctx->fp32 ? sizeof(float4) : sizeof(double4);
     hipError_t e = hipSuccess;
-    ctx->posA = dalloc<double2>(n, e);
-    ctx->posB = dalloc<double2>(n, e);
+    const size_t dim = ctx->P.dim;
+    ctx->posA = dalloc<double2>(n * (dim == 3 ? 2 : 1), e);  // 3D: one double4 per particle
+    ctx->posB = dalloc<double2>(n * (dim == 3 ? 2 : 1), e);
     ctx->vaA = dalloc<char>(n * r4, e);
     ctx->vaB = dalloc<char>(n * r4, e);
     ctx->adA = dalloc<char>(n * r4, e);
@@ -553,6 +631,10 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->pred_scratch = dalloc<char>(n * r4, e);
     ctx->pres = dalloc<char>(n * (r4 / 4), e);
     ctx->pres_scratch = dalloc<char>(n * (r4 / 4), e);
+    if (dim == 3) {
+        ctx->pw = dalloc<char>(n * (r4 / 4), e);
+        ctx->pw_scratch = dalloc<char>(n * (r4 / 4), e);
+    }
     ctx->oidA = dalloc<uint32_t>(n, e);
     ctx->oidB = dalloc<uint32_t>(n, e);
     ctx->key_next = dalloc<uint32_t>(n, e);
@@ -593,13 +675,14 @@ static rrsph_status ensure_staging(rrsph_ctx* ctx) {
     if (ctx->st_r) return RRSPH_OK;
     const size_t n = ctx->P.n;
     hipError_t e = hipSuccess;
-    ctx->st_r = dalloc<double>(2 * n, e);
-    ctx->st_v = dalloc<double>(2 * n, e);
+    const size_t dim = ctx->P.dim;
+    ctx->st_r = dalloc<double>(dim * n, e);
+    ctx->st_v = dalloc<double>(dim * n, e);
     ctx->st_rho = dalloc<double>(n, e);
     ctx->st_p = dalloc<double>(n, e);
-    ctx->st_a = dalloc<double>(2 * n, e);
+    ctx->st_a = dalloc<double>(dim * n, e);
     ctx->st_drho = dalloc<double>(n, e);
-    ctx->st_av = dalloc<double>(2 * n, e);
+    ctx->st_av = dalloc<double>(dim * n, e);
     ctx->st_mu = dalloc<double>(n, e);
     ctx->st_itype = dalloc<int32_t>(n, e);
     ctx->st_u32 = dalloc<uint32_t>(n, e);
@@ -615,8 +698,9 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
     rrsph_status st = ensure_staging(ctx);
     if (st != RRSPH_OK) return st;
     const size_t n = ctx->P.n;
-    HIP_TRY(hipMemcpyAsync(ctx->st_r, r, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->st_v, v, 2 * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const size_t dim = ctx->P.dim;
+    HIP_TRY(hipMemcpyAsync(ctx->st_r, r, dim * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->st_v, v, dim * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->st_rho, rho, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->st_itype, itype, n * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
     Ctrl c{};
@@ -626,13 +710,23 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
     *ctx->host_ctrl = c;
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    if (ctx->fp32)
+    if (dim == 3) {
+        if (ctx->fp32)
+            init3_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho,
+                                                                ctx->st_itype, stateA3<float>(ctx), ctx->key_next,
+                                                                ctx->ctrl);
+        else
+            init3_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho,
+                                                                 ctx->st_itype, stateA3<double>(ctx), ctx->key_next,
+                                                                 ctx->ctrl);
+    } else if (ctx->fp32) {
         init_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho, ctx->st_itype,
                                                            stateA<float>(ctx), ctx->key_next, ctx->ctrl);
-    else
+    } else {
         init_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->st_r, ctx->st_v, ctx->st_rho,
                                                             ctx->st_itype, stateA<double>(ctx), ctx->key_next,
                                                             ctx->ctrl);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->uploaded = true;
@@ -738,21 +832,31 @@ rrsph_status rrsph_hip_get_time(rrsph_ctx* ctx, double* time, double* dt, uint64
 rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, double* v, double* rho, double* p,
                                 int32_t* itype) {
     if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
-    const size_t n = ctx->P.n;
+    const size_t n = ctx->P.n, dim = ctx->P.dim;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    if (ctx->fp32)
+    if (dim == 3) {
+        if (ctx->fp32)
+            download3_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(
+                ctx->P, stateA3<float>(ctx), ctx->stepped ? (const float*)ctx->pres : nullptr, ctx->st_r, ctx->st_v,
+                ctx->st_rho, ctx->st_p, ctx->st_itype);
+        else
+            download3_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(
+                ctx->P, stateA3<double>(ctx), ctx->stepped ? (const double*)ctx->pres : nullptr, ctx->st_r,
+                ctx->st_v, ctx->st_rho, ctx->st_p, ctx->st_itype);
+    } else if (ctx->fp32) {
         download_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(
             ctx->P, stateA<float>(ctx), ctx->stepped ? (const float*)ctx->pres : nullptr, ctx->st_r, ctx->st_v,
             ctx->st_rho, ctx->st_p, ctx->st_itype);
-    else
+    } else {
         download_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(
             ctx->P, stateA<double>(ctx), ctx->stepped ? (const double*)ctx->pres : nullptr, ctx->st_r, ctx->st_v,
             ctx->st_rho, ctx->st_p, ctx->st_itype);
+    }
     HIP_TRY(hipGetLastError());
     if ((fields & RRSPH_FIELD_R) && r)
-        HIP_TRY(hipMemcpyAsync(r, ctx->st_r, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(r, ctx->st_r, dim * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((fields & RRSPH_FIELD_V) && v)
-        HIP_TRY(hipMemcpyAsync(v, ctx->st_v, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(v, ctx->st_v, dim * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((fields & RRSPH_FIELD_RHO) && rho)
         HIP_TRY(hipMemcpyAsync(rho, ctx->st_rho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((fields & RRSPH_FIELD_P) && p)
@@ -776,9 +880,17 @@ rrsph_status rrsph_hip_fields(rrsph_ctx* ctx, double* a, double* drho, double* a
     if (!ctx || !ctx->stepped) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no step taken yet");
     if (ctx->fused && !ctx->export_fields)
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: enable rrsph_hip_set_export_fields before stepping");
-    const size_t n = ctx->P.n;
+    const size_t n = ctx->P.n, dim = ctx->P.dim;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    if (ctx->fp32)
+    if (dim == 3 && ctx->fp32)
+        fields3_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->oidA, (const float4*)ctx->der,
+                                                              (const float4*)ctx->xs, ctx->st_a, ctx->st_drho,
+                                                              ctx->st_av, ctx->st_mu);
+    else if (dim == 3)
+        fields3_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->oidA, (const double4*)ctx->der,
+                                                               (const double4*)ctx->xs, ctx->st_a, ctx->st_drho,
+                                                               ctx->st_av, ctx->st_mu);
+    else if (ctx->fp32)
         fields_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->oidA, (const float4*)ctx->der,
                                                              (const float4*)ctx->xs, ctx->st_a, ctx->st_drho,
                                                              ctx->st_av, ctx->st_mu);
@@ -787,9 +899,9 @@ rrsph_status rrsph_hip_fields(rrsph_ctx* ctx, double* a, double* drho, double* a
                                                               (const double4*)ctx->xs, ctx->st_a, ctx->st_drho,
                                                               ctx->st_av, ctx->st_mu);
     HIP_TRY(hipGetLastError());
-    if (a) HIP_TRY(hipMemcpyAsync(a, ctx->st_a, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (a) HIP_TRY(hipMemcpyAsync(a, ctx->st_a, dim * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (drho) HIP_TRY(hipMemcpyAsync(drho, ctx->st_drho, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (av) HIP_TRY(hipMemcpyAsync(av, ctx->st_av, 2 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (av) HIP_TRY(hipMemcpyAsync(av, ctx->st_av, dim * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if (arvmu) HIP_TRY(hipMemcpyAsync(arvmu, ctx->st_mu, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return RRSPH_OK;
@@ -798,7 +910,10 @@ rrsph_status rrsph_hip_fields(rrsph_ctx* ctx, double* a, double* drho, double* a
 rrsph_status rrsph_hip_cell_keys(rrsph_ctx* ctx, uint32_t* keys) {
     if (!ctx || !ctx->uploaded || !keys) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    morton_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, ctx->oidA, ctx->st_u32);
+    if (ctx->P.dim == 3)
+        morton3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->oidA, ctx->st_u32);
+    else
+        morton_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, ctx->oidA, ctx->st_u32);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(keys, ctx->st_u32, ctx->P.n * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -814,7 +929,17 @@ rrsph_status rrsph_hip_neighbours(rrsph_ctx* ctx, uint32_t* out, uint32_t max_nb
     Ctrl zero{};
     HIP_TRY(hipMemcpyAsync(ctx->util_ctrl, &zero, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    if (ctx->fp32) {
+    if (ctx->P.dim == 3 && ctx->fp32) {
+        enqueue_util_sort<float>(ctx);
+        neighbours3_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(
+            ctx->P, stateB3<float>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch), ctx->sorted_keys,
+            ctx->LB, dout, max_nbr, ctx->util_ctrl);
+    } else if (ctx->P.dim == 3) {
+        enqueue_util_sort<double>(ctx);
+        neighbours3_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(
+            ctx->P, stateB3<double>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch), ctx->sorted_keys,
+            ctx->LB, dout, max_nbr, ctx->util_ctrl);
+    } else if (ctx->fp32) {
         enqueue_util_sort<float>(ctx);
         neighbours_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, stateB<float>(ctx, ctx->pred_scratch, ctx->pres_scratch),
                                                                  ctx->sorted_keys, ctx->LB, dout, max_nbr,
@@ -841,7 +966,11 @@ rrsph_status rrsph_hip_neighbours(rrsph_ctx* ctx, uint32_t* out, uint32_t max_nb
 rrsph_status rrsph_hip_diagnostics(rrsph_ctx* ctx, rrsph_diagnostics* out) {
     if (!ctx || !ctx->uploaded || !out) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     const uint32_t nblk = std::min<uint32_t>(kDiagMaxBlocks, grid_blocks(ctx->P.n, kBlock));
-    if (ctx->fp32)
+    if (ctx->P.dim == 3 && ctx->fp32)
+        diag_partials3<float><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA3<float>(ctx), ctx->red);
+    else if (ctx->P.dim == 3)
+        diag_partials3<double><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA3<double>(ctx), ctx->red);
+    else if (ctx->fp32)
         diag_partials<float><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA<float>(ctx), ctx->red);
     else
         diag_partials<double><<<nblk, kBlock, 0, ctx->stream>>>(ctx->P, stateA<double>(ctx), ctx->red);
@@ -876,6 +1005,7 @@ rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t n
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     if (ngauges == 0) return RRSPH_OK;
     if (ngauges > 4096) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: at most 4096 gauges");
+    if (ctx->P.dim == 3) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: gauges are 2D (x-y layers)");
     double* dx = ctx->red;  // ngauges inputs followed by ngauges results
     HIP_TRY(hipMemcpyAsync(dx, x_gauges, ngauges * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(hipMemsetAsync(dx + ngauges, 0, ngauges * sizeof(double), ctx->stream));
@@ -1008,6 +1138,7 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec& G, do
 static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
     const StepParams& P = ctx->P;
     if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint needs an FP64 context");
+    if (P.dim == 3) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint is 2D");
     if (!ctx->fast_pair || P.density_treatment != 1)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: the adjoint needs continuity density, cubic kernels, approximation 2 and no "
@@ -1103,7 +1234,8 @@ static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1
 
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
     if (!ctx || !ctx->uploaded || !eta) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
-    if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: smooth gauges need an FP64 context");
+    if (ctx->fp32 || ctx->P.dim == 3)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: smooth gauges need a 2D FP64 context");
     GaugeSpec G;
     rrsph_status st = to_gauge_spec(ctx, spec, G);
     if (st != RRSPH_OK) return st;

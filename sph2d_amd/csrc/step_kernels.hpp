@@ -184,6 +184,35 @@ __global__ void step_prologue(StepParams P, Ctrl* ctrl) {
     c.outside = 0;
 }
 
+// Cell-start table over keys 0 .. ncells (+1 sentinel entry):
+// LB[c] = first sorted index whose key >= c.  Lane s owns the cells
+// (key[s-1], key[s]]; the last lane also owns the cells after its key.
+// Short runs are written by the owning lane, long runs (empty columns above
+// the water, the domain margins) by the whole wave.  Every lane of the wave
+// calls it (idle lanes with live = false).
+__device__ __forceinline__ void fill_cell_starts(const StepParams& P, uint32_t s, bool live, uint32_t k,
+                                                 const uint32_t* __restrict__ skey, uint32_t* __restrict__ LB) {
+    uint32_t lo = s == 0 ? 0u : skey[s - 1] + 1u;
+    uint32_t hi = live ? k + 1 : lo;  // exclusive
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t val = pass == 0 ? s : P.n;
+        if (pass == 1) {
+            lo = live && s == P.n - 1 ? k + 1 : 0u;
+            hi = live && s == P.n - 1 ? P.ncells + 2 : 0u;
+        }
+        const bool small = hi - lo <= 4;
+        if (small)
+            for (uint32_t c = lo; c < hi; ++c) LB[c] = val;
+        unsigned long long big = __ballot(!small);
+        while (big) {
+            const int src = __ffsll((long long)big) - 1;
+            big &= big - 1;
+            const uint32_t blo = __shfl(lo, src), bhi = __shfl(hi, src), bval = __shfl(val, src);
+            for (uint32_t c = blo + __lane_id(); c < bhi; c += kWave) LB[c] = bval;
+        }
+    }
+}
+
 // ------------------------------------------------------------------ reorder
 // Gather the state into sorted order, predict (PredictHalfStep.cpp:16-22),
 // compute the pressure of the current density (Density.h:112) and fill the
@@ -232,29 +261,7 @@ __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctr
         B.pred[s] = pr;
         B.pres[s] = p;
     }
-    // cell-start table over keys 0 .. ncells (+1 sentinel entry): lane s
-    // owns the cells (key[s-1], key[s]]; the last lane also owns the cells
-    // after its key.  Short runs are written by the owning lane, long runs
-    // (empty columns above the water, the domain margins) by the whole wave.
-    uint32_t lo = s == 0 ? 0u : skey[s - 1] + 1u;
-    uint32_t hi = live ? k + 1 : lo;  // exclusive
-    for (int pass = 0; pass < 2; ++pass) {
-        const uint32_t val = pass == 0 ? s : P.n;
-        if (pass == 1) {
-            lo = live && s == P.n - 1 ? k + 1 : 0u;
-            hi = live && s == P.n - 1 ? P.ncells + 2 : 0u;
-        }
-        const bool small = hi - lo <= 4;
-        if (small)
-            for (uint32_t c = lo; c < hi; ++c) LB[c] = val;
-        unsigned long long big = __ballot(!small);
-        while (big) {
-            const int src = __ffsll((long long)big) - 1;
-            big &= big - 1;
-            const uint32_t blo = __shfl(lo, src), bhi = __shfl(hi, src), bval = __shfl(val, src);
-            for (uint32_t c = blo + __lane_id(); c < bhi; c += kWave) LB[c] = bval;
-        }
-    }
+    fill_cell_starts(P, s, live, k, skey, LB);
 }
 
 // ------------------------------------------------------------------ neighbour walk

@@ -119,6 +119,72 @@ def dam_break(delta: float = 0.01, tank=(3.2, 1.6), column=(1.0, 0.5), layers: i
     return Layout(prm, r, v, rho, itype)
 
 
+def _wall_plane(axis: int, level: float, outward: int, extent, delta: float, layers: int) -> np.ndarray:
+    """`layers` planes of wall particles normal to `axis` (0 x, 1 y, 2 z) on
+    the lattice of spacing delta covering `extent` ((lo, hi) per other axis),
+    the first at `level`, the next ones further out along `outward` (+1/-1);
+    odd layers are shifted by delta / 2 in both in-plane directions."""
+    others = [a for a in range(3) if a != axis]
+    pts = []
+    for layer in range(layers):
+        shift = 0.5 * delta if layer % 2 else 0.0
+        u = np.arange(extent[0][0] + shift, extent[0][1] + 1e-9, delta)
+        w = np.arange(extent[1][0] + shift, extent[1][1] + 1e-9, delta)
+        U, Wg = np.meshgrid(u, w, indexing="ij")
+        q = np.zeros((U.size, 3))
+        q[:, axis] = level + outward * layer * delta
+        q[:, others[0]] = U.ravel()
+        q[:, others[1]] = Wg.ravel()
+        pts.append(q)
+    return np.concatenate(pts)
+
+
+def dam_break_3d(delta: float = 0.02, box=(0.8, 0.6, 0.4), column=(0.3, 0.3, 0.4), layers: int = 3,
+                 jitter: float = 1e-4, seed: int = 20240604, density_treatment: int = P.DENSITY_SUMMATION,
+                 hsml_coef: float = 1.3, cfl: float = 0.2, max_neighbours: int = 128) -> Layout:
+    """3D dam break (y up): box x 0..L, y 0..H, z 0..W with 3-layer Lennard-Jones
+    walls on the floor and the four sides, a fluid block (x, y, z) = column
+    in the corner x = 0, fluid starting at the LJ zero-force distance 2h from
+    the walls, at rest with rho = rho0, cubic kernels, fixed dt = cfl h / c."""
+    L, H, W = box
+    cl, chh, cw = column
+    gap = 2.0 * hsml_coef * delta
+    nx = int(round((cl - gap) / delta)) if cl > gap else 1
+    ny = int(round(chh / delta))
+    nz = int(round((cw - 2 * gap) / delta)) + 1
+    xs = gap + delta * np.arange(nx)
+    ys = gap + delta * np.arange(ny)
+    zs = gap + delta * np.arange(nz)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    fluid = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    rng = np.random.default_rng(seed)
+    fluid = fluid + rng.uniform(-jitter, jitter, size=fluid.shape)
+    top = H
+    floor = _wall_plane(1, 0.0, -1, ((0.0, L), (0.0, W)), delta, layers)
+    left = _wall_plane(0, 0.0, -1, ((delta, top), (0.0, W)), delta, layers)
+    right = _wall_plane(0, L, +1, ((delta, top), (0.0, W)), delta, layers)
+    back = _wall_plane(2, 0.0, -1, ((delta, L - delta), (delta, top)), delta, layers)
+    front = _wall_plane(2, W, +1, ((delta, L - delta), (delta, top)), delta, layers)
+    walls = np.concatenate([floor, left, right, back, front])
+    r = np.concatenate([fluid, walls])
+    nfluid, n = len(fluid), len(fluid) + len(walls)
+    margin = (layers + 2) * delta
+    prm = _base_params(delta, hsml_coef, n, nfluid, (-margin, L + margin, -margin, H + 0.4), chh)
+    prm.dim = 3
+    prm.z_mingeom, prm.z_maxgeom = -margin, W + margin
+    prm.max_neighbours = max_neighbours
+    prm.density_treatment = density_treatment
+    prm.dt_correction_method = P.DT_CORRECTION_CONST_VALUE
+    P.fill_in_computing_params(prm)
+    P.post_fill_in_model_params(prm)
+    prm.dt = cfl * prm.hsml / prm.eos_c
+    prm.simulation_time = 5000 * prm.dt
+    v = np.zeros_like(r)
+    rho = np.full(n, prm.rho0)
+    itype = np.concatenate([np.full(nfluid, P.TYPE_WATER), np.full(n - nfluid, P.TYPE_BOUNDARY)]).astype(np.int32)
+    return Layout(prm, r, v, rho, itype)
+
+
 def hydrostatic_rho(y: np.ndarray, surface: float, rho0: float, c: float, g: float = P.G) -> np.ndarray:
     """rho from the Tait EOS at hydrostatic pressure (Piston2DGenerator.py:78-94)."""
     p = rho0 * g * np.maximum(surface - y, 0.0)

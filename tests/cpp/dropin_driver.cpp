@@ -7,7 +7,7 @@
 //
 // params.txt : one "name value" per line (the ExperimentParams field names);
 //              piston_a / piston_b take a comma-separated list
-// state.bin  : uint32 n, then r[2n], v[2n], rho[n] as float64, itype[n] as int32
+// state.bin  : uint32 n, then r[dim n], v[dim n], rho[n] as float64, itype[n] as int32
 // out.bin    : uint64 steps, float64 time, float64 dt, then r, v, rho, p (float64)
 //              and itype (int32) of the final state, then uint64 probe_step and
 //              the rho and p that the output loaders returned at that step
@@ -123,7 +123,8 @@ int run(int argc, char** argv) {
     read_raw(in, &n, 1);
     if (n != params.ntotal) throw std::runtime_error("state size differs from ntotal");
     if (params.maxn < n) params.maxn = n;
-    std::vector<double> r(2 * n), v(2 * n), rho(n);
+    const int dim = (int)params.dim;
+    std::vector<double> r(dim * n), v(dim * n), rho(n);
     std::vector<int32_t> it(n);
     read_raw(in, r.data(), r.size());
     read_raw(in, v.data(), v.size());
@@ -133,9 +134,9 @@ int run(int argc, char** argv) {
     shapes::vheap_darray_floatn r_var(params.maxn), v_var(params.maxn);
     shapes::heap_darray<double> rho_a(params.maxn), p_a(params.maxn);
     shapes::heap_darray<int> itype(params.maxn);
+    rrsph_hip::aos_to_floatn(r, n, dim, r_var);
+    rrsph_hip::aos_to_floatn(v, n, dim, v_var);
     for (uint32_t i = 0; i < n; ++i) {
-        r_var.get_flt2()(i) = {r[2 * i], r[2 * i + 1]};
-        v_var.get_flt2()(i) = {v[2 * i], v[2 * i + 1]};
         rho_a(i) = rho[i];
         itype(i) = it[i];
     }
@@ -148,8 +149,10 @@ int run(int argc, char** argv) {
     write_raw(out, &steps, 1);
     write_raw(out, &t, 1);
     write_raw(out, &params.dt, 1);
-    for (uint32_t i = 0; i < n; ++i) write_raw(out, &r_var.get_flt2()(i).x, 1), write_raw(out, &r_var.get_flt2()(i).y, 1);
-    for (uint32_t i = 0; i < n; ++i) write_raw(out, &v_var.get_flt2()(i).x, 1), write_raw(out, &v_var.get_flt2()(i).y, 1);
+    rrsph_hip::floatn_to_aos(r_var, n, dim, r);
+    rrsph_hip::floatn_to_aos(v_var, n, dim, v);
+    write_raw(out, r.data(), r.size());
+    write_raw(out, v.data(), v.size());
     for (uint32_t i = 0; i < n; ++i) write_raw(out, &rho_a(i), 1);
     for (uint32_t i = 0; i < n; ++i) write_raw(out, &p_a(i), 1);
     for (uint32_t i = 0; i < n; ++i) {

@@ -2,8 +2,8 @@
 // point uses (our own code, written from the interface alone):
 //   heap_darray<T>          -- operator()(i), size()
 //                              (sources/RR/include/RR/Memory/HeapDArray.h:130-235)
-//   vheap_darray_floatn     -- get_flt2() -> heap_darray<rr_float2>
-//                              (sources/RRSPH_PARAMS/include/RRFloatnDArray.h:7-36)
+//   vheap_darray_floatn     -- get_flt2() / get_flt3() -> heap_darray<rr_float2 / 3>
+//                              (sources/RRSPH_PARAMS/include/RRFloatnDArray.h:7-44)
 //   Output                  -- setup_output(load_r, load_itype, load_v, load_p,
 //                              load_rho), start_step, update_step, finish_step
 //                              (sources/RRSPH_COMMON/include/Output.h:11-31)
@@ -33,16 +33,23 @@ private:
 struct rr_float2 {
     double x = 0, y = 0;
 };
+struct rr_float3 {
+    double x = 0, y = 0, z = 0;
+};
 
+// holds both shapes (the reference keeps one of them in a std::variant)
 class vheap_darray_floatn {
 public:
     vheap_darray_floatn() = default;
-    explicit vheap_darray_floatn(size_t n) : a_(n) {}
+    explicit vheap_darray_floatn(size_t n) : a_(n), b_(n) {}
     heap_darray<rr_float2>& get_flt2() { return a_; }
     const heap_darray<rr_float2>& get_flt2() const { return a_; }
+    heap_darray<rr_float3>& get_flt3() { return b_; }
+    const heap_darray<rr_float3>& get_flt3() const { return b_; }
 
 private:
     heap_darray<rr_float2> a_;
+    heap_darray<rr_float3> b_;
 };
 
 using shared_floatn = std::shared_ptr<vheap_darray_floatn>;

@@ -45,7 +45,7 @@ def run(d, prm, lay, nsteps, fp32=False, probe_every=0, timeout=120):
     return subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
 
 
-def read_output(d, n):
+def read_output(d, n, dim=2):
     buf = open(os.path.join(str(d), "out.bin"), "rb").read()
     off = 0
 
@@ -58,8 +58,8 @@ def read_output(d, n):
     steps = int(take(np.uint64, 1)[0])
     t, dt = take(np.float64, 2)
     out = {"steps": steps, "time": float(t), "dt": float(dt)}
-    out["r"] = take(np.float64, 2 * n).reshape(n, 2)
-    out["v"] = take(np.float64, 2 * n).reshape(n, 2)
+    out["r"] = take(np.float64, dim * n).reshape(n, dim)
+    out["v"] = take(np.float64, dim * n).reshape(n, dim)
     out["rho"] = take(np.float64, n)
     out["p"] = take(np.float64, n)
     out["itype"] = take(np.int32, n)

@@ -95,7 +95,8 @@ def _no_device():
 @pytest.mark.parametrize(
     "change,status,text",
     [
-        ({"dim": 3}, 7, "only dim = 2"),
+        ({"dim": 4}, 1, "dim must be 2 or 3"),
+        ({"dim": 3, "density_treatment": 2}, 7, "3D supports cubic kernels"),
         ({"ntotal": 0}, 1, "invalid sizes"),
         ({"max_neighbours": 1}, 1, "invalid sizes"),
         ({"x_maxgeom": 1.0e9}, 6, "can't make grid with so many cells"),
@@ -130,12 +131,12 @@ def test_dropin_driver_reports_errors_like_the_reference(tmp_path):
         pytest.fail("dropin_driver not built (python -m sph2d_amd.build)")
     lay = G.dam_break()
     prm = lay.params
-    prm.dim = 3
+    prm.max_neighbours = 1
     res = dropin.run(tmp_path, prm, lay, nsteps=1)
     assert res.returncode == 1
-    assert "Catch exception: RRSPH_HIP: only dim = 2 is implemented" in res.stdout
+    assert "Catch exception: RRSPH_HIP: invalid sizes" in res.stdout
     if _no_device():
-        prm.dim = 2
+        prm.max_neighbours = 64
         res = dropin.run(tmp_path, prm, lay, nsteps=1)
         assert res.returncode == 1
         assert "Catch exception: RRSPH_HIP: no HIP device" in res.stdout
