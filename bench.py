@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="steps of the differentiable wave tank")
     ap.add_argument("--adjoint-every", type=int, default=100, help="checkpoint interval of the adjoint")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--steps-3d", type=int, default=20, help="steps of the 3D dam break (0: skip)")
+    ap.add_argument("--delta-3d", type=float, default=0.0025, help="spacing of the 3D dam break")
     return ap.parse_args()
 
 
@@ -70,6 +72,7 @@ def gpu_leg(layout, precision, steps, warmup, profile_steps):
     ev_ms = g.time_steps(steps)
     # per-kernel averages: HIP events around every launch on the context's stream
     per_kernel, prof_total = g.profile_steps(profile_steps)
+    sort_stats = g.sort_stats()
     t, dt, step = g.time()
     diag = g.diagnostics()
     g.close()
@@ -82,6 +85,7 @@ def gpu_leg(layout, precision, steps, warmup, profile_steps):
         "dt": dt,
         "step": step,
         "diag": diag,
+        "sort_stats": sort_stats,
     }
 
 
@@ -127,6 +131,38 @@ def adjoint_leg(steps, every):
         "backward_s": round(t2 - t1, 4),
         "grad_norm": float((grads["a"] ** 2).sum() + (grads["b"] ** 2).sum()) ** 0.5,
     }
+
+
+def leg_3d(steps, delta):
+    """3D dam break (y up): box 1.6 x 0.8 x 0.6 m, fluid block 0.4 x 0.3 x 0.6 m,
+    h = 1.3 delta, summation density, 3-layer LJ walls; FP32 and FP64."""
+    import torch
+
+    from sph2d_amd import generators as G
+    from sph2d_amd import hip as H
+
+    lay = G.dam_break_3d(delta=delta, box=(1.6, 0.8, 0.6), column=(0.4, 0.3, 0.6), jitter=0.0)
+    prm = lay.params
+    out = {"workload": "3D dam break 1.6 x 0.8 x 0.6 m box, 0.4 x 0.3 x 0.6 m fluid block, delta=%g m, h=1.3 delta, "
+                       "%d particles (%d fluid), summation density, cubic kernels, fixed dt" % (delta, prm.ntotal,
+                                                                                              prm.nfluid),
+           "particles": prm.ntotal}
+    for name, prec in (("fp32", H.FP32), ("fp64", H.FP64)):
+        g = H.HipStep(prm, precision=prec)
+        g.upload(lay.r, lay.v, lay.rho, lay.itype)
+        g.step(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        per_kernel, _ = g.profile_steps(2)
+        g.close()
+        out[name] = {"value": round(prm.ntotal * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
+                     "pair_ms": round(per_kernel["pair_kernel"], 4),
+                     "density_sum_ms": round(per_kernel["density_sum_kernel"], 4)}
+    out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
+    return out
 
 
 def cpu_leg(layout, steps):
@@ -191,6 +227,8 @@ def main():
         with open(traffic_file) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
+    r3d = leg_3d(args.steps_3d, args.delta_3d) if args.steps_3d > 0 else None
+
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_every)
@@ -229,6 +267,7 @@ def main():
         },
         "event_ms_per_step": round(r32["event_ms"] / args.steps, 5),
         "per_kernel_ms": {k: round(v, 5) for k, v in pk.items()},
+        "sort_stats": r32["sort_stats"],
         "fp64": None if r64 is None else {
             "value": round(n * args.steps / r64["wall_s"], 1),
             "ms_per_step": round(r64["wall_s"] / args.steps * 1e3, 5),
@@ -237,6 +276,7 @@ def main():
         "fp32_over_fp64": None if r64 is None else round(r64["wall_s"] / r32["wall_s"], 3),
         "adjoint_over_forward": None if adj is None else round(adj["backward_s"] / adj["forward_s"], 3),
         "adjoint": adj,
+        "dam_break_3d": r3d,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

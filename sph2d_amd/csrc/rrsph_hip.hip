@@ -147,6 +147,9 @@ struct rrsph_ctx {
     uint32_t resort_cap = kResortCap;
     bool incremental = true;            // false: every step runs the full radix sort
     bool need_full = true;              // skeyA does not describe A (after upload)
+    uint64_t launched = 0;              // steps launched by this context
+    uint64_t full_until = 0;            // after repeated re-sort overflows: full sorts until this launch count
+    uint64_t last_fallback = ~0ull;     // launch count at the last re-sort overflow
     uint64_t n_incremental = 0, n_full = 0, n_fallback = 0;
     AdjTape adj;
 };
@@ -180,6 +183,11 @@ template <class R> StateB3<R> stateB3(rrsph_ctx* c, void* pred, void* pw, void* 
 }
 
 uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
+
+// steps that use the full radix sort after incremental re-sort overflows
+// within kFallbackWindow launched steps of each other
+constexpr uint64_t kFullSortRun = 64;
+constexpr uint64_t kFallbackWindow = 8;
 
 // Launch helper that optionally records HIP events around the launch.
 struct Launcher {
@@ -743,10 +751,11 @@ static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
         if (st != RRSPH_OK) return st;
     }
     for (uint32_t k = 0; k < nsteps; ++k) {
-        const bool full = ctx->need_full || !ctx->incremental;
+        const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
         if (ctx->use_graph) HIP_TRY(hipGraphLaunch(full ? ctx->gexec_full : ctx->gexec, ctx->stream));
         else enqueue_step(ctx, nullptr, full);
         ++(full ? ctx->n_full : ctx->n_incremental);
+        ++ctx->launched;
         ctx->need_full = false;
     }
     HIP_TRY(hipGetLastError());
@@ -764,6 +773,13 @@ static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t
     *left = 0;
     if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
     ++ctx->n_fallback;
+    // overflows in close succession mean many particles change cell every
+    // step: keep to the full sort for a while instead of paying a host round
+    // trip per step, then try again (a lone overflow, e.g. the first step
+    // after an upload, costs one round trip)
+    if (ctx->last_fallback != ~0ull && ctx->launched - ctx->last_fallback <= kFallbackWindow)
+        ctx->full_until = ctx->launched + kFullSortRun;
+    ctx->last_fallback = ctx->launched;
     *left = nsteps - (uint32_t)(committed_steps(c) - first);
     c.halted = 0;
     c.status = 0;
@@ -1046,9 +1062,10 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     HIP_TRY(hipEventRecord(t0, ctx->stream));
     const uint64_t first = committed_steps(*ctx->host_ctrl);
     for (uint32_t k = 0; k < nsteps; ++k) {
-        const bool full = ctx->need_full || !ctx->incremental;
+        const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
         enqueue_step(ctx, &prof, full);
         ++(full ? ctx->n_full : ctx->n_incremental);
+        ++ctx->launched;
         ctx->need_full = false;
     }
     HIP_TRY(hipEventRecord(t1, ctx->stream));

@@ -137,11 +137,23 @@ __device__ __forceinline__ void for_each_neighbour3(const StepParams& P, uint32_
     }
 }
 
-// Two-phase walk of the 3D pair pass (as collect_neighbours in 2D): hits go
-// to the lane's column of the LDS list, flush(nh) consumes them.
+// Two-phase walk of the 3D pair pass: hits go to the lane's column of the
+// LDS list, flush(nh) consumes them.  A particle has ~75 neighbours in 3D,
+// so lists fill during the walk; to keep the flushes of a wave together the
+// walk is wave-uniform: a uniform loop over the nine columns and, in each,
+// over the wave's longest range (shorter ranges masked), with the flush
+// decided by a ballot.  Every lane of the wave calls it; lanes with
+// active = false walk empty ranges.  Same neighbour set, visiting order and
+// overflow rule as for_each_neighbour3.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, off));
+    return v;
+}
+
 template <uint32_t kChunk, class F>
-__device__ __forceinline__ void collect_neighbours3(const StepParams& P, uint32_t cap, uint32_t s, uint32_t key,
-                                                    double4 pj, const double4* __restrict__ pos,
+__device__ __forceinline__ void collect_neighbours3(const StepParams& P, bool active, uint32_t cap, uint32_t s,
+                                                    uint32_t key, double4 pj, const double4* __restrict__ pos,
                                                     const uint32_t* __restrict__ LB, uint32_t* __restrict__ hits,
                                                     bool& overflow, F&& flush) {
     static_assert(kChunk >= 2 * kCollect, "hit list too short");
@@ -155,24 +167,26 @@ __device__ __forceinline__ void collect_neighbours3(const StepParams& P, uint32_
 #pragma unroll 1
     for (int c = 0; c < 9; ++c) {
         const int x = cx + c / 3 - 1, y = cy + c % 3 - 1;
-        if (x < 0 || x >= P.ncx || y < 0 || y >= P.ncy) continue;
-        const uint32_t col = ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz;
-        const uint32_t b = LB[col + z0], e = LB[col + z1 + 1];
+        const bool ok = active && x >= 0 && x < P.ncx && y >= 0 && y < P.ncy;
+        const uint32_t col = ok ? ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz : 0u;
+        const uint32_t b = ok ? LB[col + z0] : 0u;
+        const uint32_t len = ok ? LB[col + z1 + 1] - b : 0u;
+        const uint32_t lmax = wave_max_u32(len);
 #pragma unroll 1
-        for (uint32_t i = b; i < e; i += kCollect) {
+        for (uint32_t i = 0; i < lmax; i += kCollect) {
             double4 q[kCollect];
 #pragma unroll
-            for (uint32_t u = 0; u < kCollect; ++u) q[u] = pos[min(i + u, e - 1)];
+            for (uint32_t u = 0; u < kCollect; ++u) q[u] = pos[len ? b + min(i + u, len - 1) : s];
             bool hit[kCollect];
 #pragma unroll
             for (uint32_t u = 0; u < kCollect; ++u) {
                 const double dx = __dsub_rn(q[u].x, pj.x), dy = __dsub_rn(q[u].y, pj.y), dz = __dsub_rn(q[u].z, pj.z);
-                hit[u] = (i + u < e) & (i + u != s) & (dist2_exact3(dx, dy, dz) < P.cut2);
+                hit[u] = (i + u < len) & (b + i + u != s) & (dist2_exact3(dx, dy, dz) < P.cut2);
             }
             if (done + nh + kCollect <= cap) {
 #pragma unroll
                 for (uint32_t u = 0; u < kCollect; ++u) {
-                    hits[nh * kBlock] = i + u;
+                    hits[nh * kBlock] = b + i + u;
                     nh += hit[u] ? 1u : 0u;
                 }
             } else {
@@ -180,21 +194,21 @@ __device__ __forceinline__ void collect_neighbours3(const StepParams& P, uint32_
                 for (uint32_t u = 0; u < kCollect; ++u) {
                     if (!hit[u]) continue;
                     if (done + nh < cap) {
-                        hits[nh * kBlock] = i + u;
+                        hits[nh * kBlock] = b + i + u;
                         ++nh;
                     } else {
                         overflow = true;
                     }
                 }
             }
-            if (nh + kCollect > kChunk) {
+            if (__any(nh + kCollect > kChunk)) {  // wave-uniform
                 flush(nh);
                 done += nh;
                 nh = 0;
             }
         }
     }
-    if (nh) flush(nh);
+    flush(nh);
 }
 
 // ------------------------------------------------------------------ summation density
@@ -374,9 +388,10 @@ __global__ void __launch_bounds__(kBlock) pair3_kernel(StepParams P, Ctrl* __res
         oid = B.oid[s];
     }
     const int tj = (int)pj.w;
-    if (live && tj != 0) {
+    {
         uint32_t* my = hits + threadIdx.x;
         auto flush = [&](uint32_t nh) {
+            if (nh == 0) return;
             const uint32_t last = nh - 1;
             uint32_t i1 = my[min(1u, last) * kBlock];
             const uint32_t i0 = my[0];
@@ -398,7 +413,8 @@ __global__ void __launch_bounds__(kBlock) pair3_kernel(StepParams P, Ctrl* __res
                                    prj, pwj, tj);
             }
         };
-        collect_neighbours3<kPairChunk<R>>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, my, overflow, flush);
+        collect_neighbours3<kPairChunk<R>>(P, live && tj != 0, P.max_nbr - 1, live ? s : 0u, live ? skey[s] : 0u,
+                                           pj, B.pos, LB, my, overflow, flush);
     }
     const R m = (R)P.mass;
     const R ax = ljx + m * sx;

@@ -1,7 +1,11 @@
 """Profiling driver: runs a few RRSPH_HIP steps of a workload, nothing else.
 
-    python -m sph2d_amd.profile_step [--steps 3] [--warmup 2] [--fp64] [--delta D]
+    python -m sph2d_amd.profile_step [--steps 3] [--warmup 2] [--fp64] [--delta D] [--dam-break-3d]
     python -m sph2d_amd.profile_step --copy-calibration
+
+Default workload: the 2D piston flume at ~1M particles.  --dam-break-3d:
+the 3D dam break (1.6 x 0.8 x 0.6 m box, 0.4 x 0.3 x 0.6 m fluid block,
+delta 0.0025 m unless --delta is given, ~6.6M particles).
 
 Meant to sit under rocprofv3 (kernel traces, PMC counter passes).  The
 calibration mode copies a 256 MiB device buffer five times so that the
@@ -23,6 +27,7 @@ def main():
     ap.add_argument("--fp64", action="store_true")
     ap.add_argument("--delta", type=float, default=None)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--dam-break-3d", action="store_true")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -38,7 +43,10 @@ def main():
     from sph2d_amd import generators as G
     from sph2d_amd import hip as H
 
-    lay = G.piston_tank(delta=args.delta or G.c1_1m_delta())
+    if args.dam_break_3d:
+        lay = G.dam_break_3d(delta=args.delta or 0.0025, box=(1.6, 0.8, 0.6), column=(0.4, 0.3, 0.6), jitter=0.0)
+    else:
+        lay = G.piston_tank(delta=args.delta or G.c1_1m_delta())
     g = H.HipStep(lay.params, precision=H.FP64 if args.fp64 else H.FP32)
     g.upload(lay.r, lay.v, lay.rho, lay.itype)
     if args.warmup:
