@@ -75,6 +75,12 @@ def gpu_leg(layout, precision, steps, warmup, profile_steps):
     sort_stats = g.sort_stats()
     t, dt, step = g.time()
     diag = g.diagnostics()
+    # one more step with the pair-pass fields exported, for the FP32-vs-FP64 comparison
+    g.set_export_fields(True)
+    g.step(1)
+    state = g.download()
+    state["a"] = g.fields()["a"]
+    state["step"] = g.time()[2]
     g.close()
     return {
         "wall_s": wall_s,
@@ -86,6 +92,7 @@ def gpu_leg(layout, precision, steps, warmup, profile_steps):
         "step": step,
         "diag": diag,
         "sort_stats": sort_stats,
+        "state": state,
     }
 
 
@@ -147,6 +154,7 @@ def leg_3d(steps, delta):
                        "%d particles (%d fluid), summation density, cubic kernels, fixed dt" % (delta, prm.ntotal,
                                                                                               prm.nfluid),
            "particles": prm.ntotal}
+    states = {}
     for name, prec in (("fp32", H.FP32), ("fp64", H.FP64)):
         g = H.HipStep(prm, precision=prec)
         g.upload(lay.r, lay.v, lay.rho, lay.itype)
@@ -157,12 +165,33 @@ def leg_3d(steps, delta):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         per_kernel, _ = g.profile_steps(2)
+        g.set_export_fields(True)
+        g.step(1)
+        states[name] = g.download()
+        states[name]["a"] = g.fields()["a"]
+        states[name]["step"] = g.time()[2]
         g.close()
         out[name] = {"value": round(prm.ntotal * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
                      "pair_ms": round(per_kernel["pair_kernel"], 4),
                      "density_sum_ms": round(per_kernel["density_sum_kernel"], 4)}
     out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
+    out["fp32_error_vs_fp64"] = precision_error(states["fp32"], states["fp64"], delta)
     return out
+
+
+def precision_error(s32, s64, delta):
+    """Error of the FP32 run against the FP64 run after the same steps from
+    the same initial state: max relative density error, max position
+    difference (m and in units of the particle spacing), and the max
+    acceleration difference relative to the largest FP64 acceleration."""
+    import numpy as np
+
+    rho = float(np.max(np.abs(s32["rho"] - s64["rho"]) / s64["rho"]))
+    dr = float(np.max(np.abs(s32["r"] - s64["r"])))
+    a64 = np.linalg.norm(s64["a"], axis=1)
+    da = float(np.max(np.linalg.norm(s32["a"] - s64["a"], axis=1)) / max(float(a64.max()), 1e-300))
+    return {"steps": int(s64["step"]), "rho_rel_max": rho, "r_max_m": dr, "r_max_over_delta": dr / delta,
+            "a_max_rel_to_scale": da}
 
 
 def cpu_leg(layout, steps):
@@ -274,6 +303,7 @@ def main():
             "per_kernel_ms": {k: round(v, 5) for k, v in r64["per_kernel_ms"].items()},
         },
         "fp32_over_fp64": None if r64 is None else round(r64["wall_s"] / r32["wall_s"], 3),
+        "fp32_error_vs_fp64": None if r64 is None else precision_error(r32["state"], r64["state"], delta),
         "adjoint_over_forward": None if adj is None else round(adj["backward_s"] / adj["forward_s"], 3),
         "adjoint": adj,
         "dam_break_3d": r3d,

@@ -13,8 +13,9 @@
 // Requirements on the template arguments (all met by the reference's types):
 //   Params      : the fields of ExperimentParams (sources/RRSPH_PARAMS/include/Params.h:8-194)
 //   Output      : setup_output(load_r, load_itype, load_v, load_p, load_rho),
-//                 start_step(t), update_step(t, step), finish_step()
-//                 (sources/RRSPH_COMMON/include/Output.h:11-31)
+//                 start_step(t), update_step(t, step), finish_step(),
+//                 crash_dump(r, itype, v, rho, p, step, time)
+//                 (sources/RRSPH_COMMON/include/Output.h:11-56)
 //   FloatnArray : vheap_darray_floatn -- get_flt2() / get_flt3() returning an
 //                 indexable array of {x, y} / {x, y, z}; constructible from a size
 //   FloatArray / IntArray : heap_darray<rr_float> / heap_darray<rr_int>
@@ -249,14 +250,31 @@ void hip_time_integration(Params& params, Output& output, FloatnArray& r_var, Fl
                         [=]() { return load_floatn(true); }, [=]() { return load_scalar(RRSPH_FIELD_P); },
                         [=]() { return load_scalar(RRSPH_FIELD_RHO); });
 
+    // The in-bounds check of every step (RRSPHOutput::update_step,
+    // Output.cpp:366-412) runs on the device: the library halts with the
+    // reference's error under STOP.  update_step therefore runs with the
+    // host check off (it would download r and itype every step), and the
+    // crash dump the reference writes before rethrowing is written here.
+    const bool check = params.consistency_check;
     double time = params.start_simulation_time, dt = params.dt;
     uint64_t step = 0;
     while (time <= params.simulation_time) {
         output.start_step(time);
-        ctx.step(1);
+        try {
+            ctx.step(1);
+        } catch (const Error& e) {
+            if (e.status == RRSPH_ERR_OUTSIDE_BOUNDARIES && params.use_crash_dump) {
+                ctx.time(time, dt, step);
+                output.crash_dump(load_floatn(false), load_itype(), load_floatn(true), load_scalar(RRSPH_FIELD_RHO),
+                                  load_scalar(RRSPH_FIELD_P), step, time);
+            }
+            throw;
+        }
         ctx.time(time, dt, step);
         params.dt = dt;
+        params.consistency_check = false;
         output.update_step(time, step);
+        params.consistency_check = check;
         output.finish_step();
     }
     // leave the final state in the caller's arrays, as RRSPH_OMP does

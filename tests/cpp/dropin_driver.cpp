@@ -46,7 +46,7 @@ struct Params {
     double average_velocity_coef = 0.3;
     unsigned boundary_treatment = 0, dt_correction_method = 0;
     double dt = 0, CFL_coef = 0;
-    bool consistency_check = true;
+    bool consistency_check = true, use_crash_dump = true;
     unsigned consistency_treatment = 0;
     unsigned nwm = 0, nwm_direction = 0, nwm_particles_start = 0, nwm_particles_end = 0;
     double nwm_time_start = 0, nwm_freq = 0, nwm_wave_number = 0, nwm_wave_magnitude = 0, nwm_phase = 0;
@@ -83,7 +83,7 @@ void load_params(const char* path, Params& p, rrsph_hip::Options& o) {
     F(artificial_viscosity); F(artificial_viscosity_skf); F(artificial_shear_visc); F(artificial_bulk_visc);
     F(average_velocity); F(average_velocity_skf); F(average_velocity_coef);
     F(boundary_treatment); F(dt_correction_method); F(dt); F(CFL_coef);
-    F(consistency_check); F(consistency_treatment);
+    F(consistency_check); F(consistency_treatment); F(use_crash_dump);
     F(nwm); F(nwm_direction); F(nwm_particles_start); F(nwm_particles_end);
     F(nwm_time_start); F(nwm_freq); F(nwm_wave_number); F(nwm_wave_magnitude); F(nwm_phase);
     F(start_simulation_time); F(simulation_time);
@@ -141,7 +141,13 @@ int run(int argc, char** argv) {
         itype(i) = it[i];
     }
 
-    rrsph_hip::hip_time_integration(params, output, r_var, v_var, rho_a, p_a, itype, opt);
+    try {
+        rrsph_hip::hip_time_integration(params, output, r_var, v_var, rho_a, p_a, itype, opt);
+    } catch (...) {
+        if (output.crashed)
+            std::cout << "crash dump at step " << output.crash_step << " time " << output.crash_time << std::endl;
+        throw;
+    }
 
     std::ofstream out(argv[3], std::ios::binary);
     const uint64_t steps = output.last_step;

@@ -84,6 +84,15 @@ public:
         }
     }
     void finish_step() { ++finished; }
+    void crash_dump(shared_floatn r, shared_darray<int> itype, shared_floatn v, shared_darray<double> rho,
+                    shared_darray<double> p, size_t step, double time) {
+        crash_step = step;
+        crash_time = time;
+        crashed = r && itype && v && rho && p;
+    }
+    bool crashed = false;
+    size_t crash_step = 0;
+    double crash_time = 0;
 
     size_t probe_every = 0;
     std::vector<double> starts, times;

@@ -57,3 +57,24 @@ def test_dropin_piston_dynamic_dt(tmp_path):
     assert abs(out["dt"] - ref["dt"]) <= 1e-12 * ref["dt"]
     assert np.max(np.abs(out["r"] - ref["r"])) < 1e-11
     assert np.max(np.abs(out["rho"] - ref["rho"]) / ref["rho"]) < 1e-11
+
+
+def test_dropin_stop_crash_dump(tmp_path):
+    """A fluid particle thrown out of the domain: the device check halts the
+    step (STOP), the adapter writes the crash dump through the output and
+    rethrows the reference's message (Output.cpp:395-412, ConsistencyCheck.cpp:184-195)."""
+    lay = G.dam_break()
+    prm = lay.params
+    v = lay.v.copy()
+    v[0] = [1.0e4, 0.0]
+    lay2 = G.Layout(prm, lay.r, v, lay.rho, lay.itype)
+    # the step the oracle leaves the domain on (the particle stays inside
+    # after the first half step and crosses the floor on the second)
+    ora = oracle.Oracle(prm)
+    ora.set_state(lay2.r, lay2.v, lay2.rho, lay2.itype, time=prm.start_simulation_time, dt=prm.dt)
+    crash = next(k for k in range(1, 6) if ora.step(1) != 0)
+    t_crash = ora.state()["time"]
+    res = dropin.run(tmp_path, prm, lay2, nsteps=5)
+    assert res.returncode == 1, res.stdout
+    assert f"crash dump at step {crash} time {t_crash:g}" in res.stdout, res.stdout
+    assert "Catch exception: Encounter particle outside boundaries" in res.stdout, res.stdout
