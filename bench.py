@@ -165,6 +165,7 @@ def leg_3d(steps, delta):
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         per_kernel, _ = g.profile_steps(2)
+        sort_stats = g.sort_stats()
         g.set_export_fields(True)
         g.step(1)
         states[name] = g.download()
@@ -173,7 +174,7 @@ def leg_3d(steps, delta):
         g.close()
         out[name] = {"value": round(prm.ntotal * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
                      "pair_ms": round(per_kernel["pair_kernel"], 4),
-                     "density_sum_ms": round(per_kernel["density_sum_kernel"], 4)}
+                     "density_sum_ms": round(per_kernel["density_sum_kernel"], 4), "sort_stats": sort_stats}
     out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
     out["fp32_error_vs_fp64"] = precision_error(states["fp32"], states["fp64"], delta)
     return out

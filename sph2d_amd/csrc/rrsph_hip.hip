@@ -149,7 +149,9 @@ struct rrsph_ctx {
     bool need_full = true;              // skeyA does not describe A (after upload)
     uint64_t launched = 0;              // steps launched by this context
     uint64_t full_until = 0;            // after repeated re-sort overflows: full sorts until this launch count
-    uint64_t last_fallback = ~0ull;     // launch count at the last re-sort overflow
+    uint64_t last_fallback = ~0ull;     // step index of the last re-sort overflow
+    uint64_t full_run_end = 0;          // step index where the last run of full sorts ended
+    uint64_t full_run = 0;              // length of the last run of full sorts
     uint64_t n_incremental = 0, n_full = 0, n_fallback = 0;
     AdjTape adj;
 };
@@ -184,9 +186,11 @@ template <class R> StateB3<R> stateB3(rrsph_ctx* c, void* pred, void* pw, void* 
 
 uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
 
-// steps that use the full radix sort after incremental re-sort overflows
-// within kFallbackWindow launched steps of each other
+// After two incremental re-sort overflows within kFallbackWindow steps of
+// each other, the next kFullSortRun steps use the full radix sort; an
+// overflow soon after such a run ends doubles the run (up to kFullSortRunMax).
 constexpr uint64_t kFullSortRun = 64;
+constexpr uint64_t kFullSortRunMax = 4096;
 constexpr uint64_t kFallbackWindow = 8;
 
 // Launch helper that optionally records HIP events around the launch.
@@ -740,6 +744,10 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
     ctx->uploaded = true;
     ctx->stepped = false;
     ctx->need_full = true;
+    // step indices restart: forget the overflow history
+    ctx->full_until = ctx->launched;
+    ctx->last_fallback = ~0ull;
+    ctx->full_run = ctx->full_run_end = 0;
     return RRSPH_OK;
 }
 
@@ -773,14 +781,21 @@ static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t
     *left = 0;
     if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
     ++ctx->n_fallback;
-    // overflows in close succession mean many particles change cell every
+    // Overflows in close succession mean many particles change cell every
     // step: keep to the full sort for a while instead of paying a host round
-    // trip per step, then try again (a lone overflow, e.g. the first step
-    // after an upload, costs one round trip)
-    if (ctx->last_fallback != ~0ull && ctx->launched - ctx->last_fallback <= kFallbackWindow)
-        ctx->full_until = ctx->launched + kFullSortRun;
-    ctx->last_fallback = ctx->launched;
-    *left = nsteps - (uint32_t)(committed_steps(c) - first);
+    // trip (and the no-op tail of the launched batch) per step, then try
+    // again.  A lone overflow, e.g. the first step after an upload, costs one
+    // round trip.  Distances are in committed steps, not launches: a halted
+    // batch still counts its no-op launches.
+    const uint64_t at = committed_steps(c);  // the step that overflowed
+    if (ctx->last_fallback != ~0ull && at - ctx->last_fallback <= kFallbackWindow) {
+        const bool again = ctx->full_run && at - ctx->full_run_end <= kFallbackWindow;
+        ctx->full_run = again ? std::min(2 * ctx->full_run, kFullSortRunMax) : kFullSortRun;
+        ctx->full_until = ctx->launched + ctx->full_run;
+        ctx->full_run_end = at + ctx->full_run;
+    }
+    ctx->last_fallback = at;
+    *left = nsteps - (uint32_t)(at - first);
     c.halted = 0;
     c.status = 0;
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
