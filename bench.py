@@ -4,7 +4,7 @@
 
 Workload (default): the 2D piston wave flume at ~1M particles ("C1-1M"):
 20 m x 0.5 m of water at spacing 0.0031623 m (999,350 fluid particles,
-1,020,266 in total with 3-layer Lennard-Jones walls), continuity density,
+1,020,263 in total with 3-layer Lennard-Jones walls), continuity density,
 Tait EOS, artificial viscosity, XSPH, piston x(t) = 0.05 sin(2 pi t / 1.2),
 dynamic CFL dt from the max-|a| / max-mu reductions.  A "step" is one whole
 time step (sort, predict, density, forces, dt, integrate) over all particles.
@@ -15,7 +15,10 @@ resident in HBM; the FP64 path is timed beside it.  `roofline` is the
 dominant kernel's algorithmic bytes per launch over its average duration
 (HIP events on the context's stream); `cpu_baseline` is the FP64 CPU oracle
 (a port of the reference's RRSPH_OMP step) on a bounded sample of the same
-workload, on this host's cores.
+workload, on this host's cores.  Beside the headline: the 3D dam break
+(6.6M particles, FP32 and FP64), the 2D differentiable wave tank and the 3D
+piston basin with 8 paddles (adjoint / forward cost, gradient against
+central differences).
 """
 import argparse
 import json
@@ -47,6 +50,10 @@ def parse():
     ap.add_argument("--adjoint-steps", type=int, default=2000, help="steps of the differentiable wave tank")
     ap.add_argument("--adjoint-every", type=int, default=100, help="checkpoint interval of the adjoint")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint3d-steps", type=int, default=100, help="steps of the 3D basin adjoint (0: skip)")
+    ap.add_argument("--adjoint3d-every", type=int, default=50)
+    ap.add_argument("--adjoint3d-delta", type=float, default=0.01)
+    ap.add_argument("--adjoint3d-fd", type=int, default=8, help="coefficients checked by central differences")
     ap.add_argument("--steps-3d", type=int, default=20, help="steps of the 3D dam break (0: skip)")
     ap.add_argument("--delta-3d", type=float, default=0.0025, help="spacing of the 3D dam break")
     return ap.parse_args()
@@ -137,6 +144,64 @@ def adjoint_leg(steps, every):
         "taped_forward_s": round(t1 - t0, 4),
         "backward_s": round(t2 - t1, 4),
         "grad_norm": float((grads["a"] ** 2).sum() + (grads["b"] ** 2).sum()) ** 0.5,
+    }
+
+
+def adjoint3d_leg(steps, every, delta, nfd):
+    """The differentiable 3D piston basin (6 m x 2 m, depth 0.5 m, 8 paddles
+    x 4 harmonics = 64 coefficients), FP64: plain forward steps, the taped
+    forward and the reverse pass on the wall clock, and the gradient of the
+    gauge loss against central differences on `nfd` sampled coefficients.
+    The 4 x 4 gauge grid sits within the distance sound travels in the run,
+    so every gauge sees the paddles."""
+    import numpy as np
+    import torch
+
+    from sph2d_amd import generators as G
+    from sph2d_amd import hip as H
+    from sph2d_amd.adjoint import DifferentiableRun, central_difference
+
+    lay = G.piston_basin_3d(delta=delta)
+    prm = lay.params
+    reach = max(prm.eos_c * steps * prm.dt, 8 * delta)
+    xs = [2 * delta + reach * (i + 1) / 5 for i in range(4)]
+    zs = [0.25, 0.75, 1.25, 1.75]
+    gx, gz = [x for x in xs for _ in zs], [z for _ in xs for z in zs]
+    spec = H.gauge_spec(gx, radius=3 * delta, tau=delta / 4, y_ref=prm.depth, zs=gz)
+    g = H.HipStep(prm, precision=H.FP64)
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    g.upload(lay.r, lay.v, lay.rho, lay.itype)
+    g.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(steps)
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t0
+    run = DifferentiableRun(g, dict(lay.state), steps, every, spec, prm.piston_omega)
+    t0 = time.perf_counter()
+    eta = run.forward(a, b)
+    t1 = time.perf_counter()
+    loss = lambda e: float(np.sum((e - prm.depth) ** 2))
+    grads = run.backward(2.0 * (eta - prm.depth), len(a))
+    t2 = time.perf_counter()
+    errs = []
+    rng = np.random.default_rng(1)
+    for j in rng.choice(2 * len(a), size=nfd, replace=False):
+        which, k = ("a", int(j)) if j < len(a) else ("b", int(j) - len(a))
+        fd = central_difference(run, a, b, loss, which, k, 1e-6)
+        errs.append({"coef": f"{which}[{k}]", "adjoint": grads[which][k], "fd": fd,
+                     "rel_err": abs(grads[which][k] - fd) / max(abs(fd), 1e-300)})
+    g.close()
+    return {
+        "workload": "3D piston basin 6 m x 2 m, depth 0.5 m, delta %g m, %d particles (%d fluid), 8 paddles x 4 "
+                    "harmonics, 4 x 4 gauges, %d steps, checkpoint every %d, FP64" % (delta, prm.ntotal, prm.nfluid,
+                                                                                     steps, every),
+        "forward_s": round(fwd, 4),
+        "taped_forward_s": round(t1 - t0, 4),
+        "backward_s": round(t2 - t1, 4),
+        "adjoint_over_forward": round((t2 - t1) / fwd, 3),
+        "fd_check": errs,
+        "fd_max_rel_err": max(e["rel_err"] for e in errs) if errs else None,
     }
 
 
@@ -262,6 +327,9 @@ def main():
     adj = None
     if not args.no_adjoint:
         adj = adjoint_leg(args.adjoint_steps, args.adjoint_every)
+    adj3 = None
+    if args.adjoint3d_steps > 0:
+        adj3 = adjoint3d_leg(args.adjoint3d_steps, args.adjoint3d_every, args.adjoint3d_delta, args.adjoint3d_fd)
 
     cpu = None
     if not args.no_cpu:
@@ -307,6 +375,7 @@ def main():
         "fp32_error_vs_fp64": None if r64 is None else precision_error(r32["state"], r64["state"], delta),
         "adjoint_over_forward": None if adj is None else round(adj["backward_s"] / adj["forward_s"], 3),
         "adjoint": adj,
+        "adjoint_3d": adj3,
         "dam_break_3d": r3d,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -48,7 +48,7 @@ typedef enum rrsph_precision {
  * ones Input.cpp:93-138 computes. */
 typedef struct rrsph_params_c {
     uint32_t struct_size;          /* = sizeof(rrsph_params_c) */
-    int32_t dim;                   /* 2 (3 reserved) */
+    int32_t dim;                   /* 2 or 3 */
     uint32_t ntotal;
     uint32_t nfluid;
     uint32_t max_neighbours;
@@ -110,6 +110,14 @@ typedef struct rrsph_params_c {
     int32_t dt_dynamic_mode;       /* 0: RRSPH_OMP OpenMP build (reductions off); 1: max |a|, max mu over all particles */
     int32_t precision;             /* rrsph_precision */
     double start_time;
+
+    /* Segmented Fourier piston (nwm = 6, 3D): piston_segments paddles across
+     * z in [piston_seg_zmin, piston_seg_zmax); a wave-maker particle at z moves
+     * with paddle floor((z - zmin) * segments / (zmax - zmin)) (clamped).
+     * piston_a / piston_b hold piston_nharm harmonics per paddle, paddle-major
+     * (segments * nharm <= RRSPH_MAX_HARMONICS).  0 or 1: one piston. */
+    int32_t piston_segments;
+    double piston_seg_zmin, piston_seg_zmax;
 } rrsph_params_c;
 
 typedef struct rrsph_ctx rrsph_ctx;
@@ -191,10 +199,11 @@ rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t n
 
 /* ---- differentiation ------------------------------------------------- */
 /* No reference counterpart: the reference has no derivative of its step.
- * The adjoint differentiates the FP64 step (continuity density, cubic
- * kernels, pressure, artificial viscosity, Lennard-Jones walls, XSPH,
- * boundary motion with the Fourier piston, nwm = 6) with dt independent of
- * the state (dt_correction_method 0, or 2 with dt_dynamic_mode 0). */
+ * The adjoint differentiates the FP64 step, 2D or 3D (continuity density,
+ * cubic kernels, pressure, artificial viscosity, Lennard-Jones walls, XSPH,
+ * boundary motion with the Fourier piston, nwm = 6, one piston or paddles
+ * in 3D) with dt independent of the state (dt_correction_method 0, or 2
+ * with dt_dynamic_mode 0). */
 #define RRSPH_MAX_GAUGES 64
 typedef struct rrsph_gauge_spec {
     uint32_t n;                    /* gauges, at most RRSPH_MAX_GAUGES */
@@ -203,6 +212,7 @@ typedef struct rrsph_gauge_spec {
     double tau;                    /* softness; eta -> max y in the window as tau -> 0 */
     double y_ref;                  /* shift of the exponent, keep (y - y_ref) / tau below ~700 */
     double x[RRSPH_MAX_GAUGES];
+    double z[RRSPH_MAX_GAUGES];    /* 3D: gauge at (x, z), weight (1 - ((x - x_g)^2 + (z - z_g)^2) / R^2)^3 */
 } rrsph_gauge_spec;
 
 /* Smooth surrogate of the gauge elevation on the current state:
@@ -210,7 +220,9 @@ typedef struct rrsph_gauge_spec {
  * over particles of type ptype (0 where the window is empty). */
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta);
 /* Replace the Fourier piston coefficients (nwm = 6):
- * x(t) = sum_k a_k sin(k w t) + b_k cos(k w t), k = 1 .. nharm. */
+ * x(t) = sum_k a_k sin(k w t) + b_k cos(k w t), k = 1 .. nharm; with
+ * piston_segments paddles, a and b hold nharm harmonics per paddle,
+ * paddle-major (segments * nharm entries). */
 rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b);
 /* Differentiable run from the current state: nsteps steps, a checkpoint
  * every checkpoint_every steps (device memory), the smooth gauges after
@@ -218,9 +230,10 @@ rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, 
 rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                                        const rrsph_gauge_spec* spec, double* eta);
 /* Reverse pass of the last adjoint_forward for eta_bar = dL/d eta (same
- * layout): gradients with respect to the piston coefficients (nharm each)
- * and, when the pointers are not NULL, with respect to the initial state
- * (original particle order; r, v as x,y pairs).  Replays each checkpoint
+ * layout): gradients with respect to the piston coefficients (segments *
+ * nharm each, the layout of set_piston) and, when the pointers are not
+ * NULL, with respect to the initial state (original particle order; r, v as
+ * dim-tuples).  Replays each checkpoint
  * segment forward and runs the step adjoints backward; leaves the state as
  * adjoint_forward left it. */
 rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,

@@ -52,6 +52,8 @@ class OracleParams(ctypes.Structure):
         ("nwm_phase", ctypes.c_double),
         ("piston_nharm", ctypes.c_int32), ("piston_omega", ctypes.c_double),
         ("piston_a", ctypes.c_double * MAX_HARMONICS), ("piston_b", ctypes.c_double * MAX_HARMONICS),
+        ("piston_segments", ctypes.c_int32), ("piston_seg_zmin", ctypes.c_double),
+        ("piston_seg_zmax", ctypes.c_double),
         ("boundary_motion", ctypes.c_int32), ("dt_dynamic_mode", ctypes.c_int32),
         ("num_threads", ctypes.c_int32),
     ]
@@ -129,7 +131,8 @@ def from_params(p, num_threads: int = 0) -> OracleParams:
     """Build the oracle's struct from a :class:`sph2d_amd.params.ExperimentParams`."""
     o = OracleParams()
     for name, _t in OracleParams._fields_:
-        if name in ("piston_a", "piston_b", "piston_nharm", "num_threads", "eos_c_sqr", "eos_c"):
+        if name in ("piston_a", "piston_b", "piston_nharm", "num_threads", "eos_c_sqr", "eos_c", "piston_segments",
+                    "piston_seg_zmin", "piston_seg_zmax"):
             continue
         val = getattr(p, name)
         setattr(o, name, type(getattr(o, name))(val))
@@ -137,7 +140,11 @@ def from_params(p, num_threads: int = 0) -> OracleParams:
     o.eos_c = p.eos_c
     o.boundary_motion = int(p.boundary_motion)
     o.dt_dynamic_mode = int(p.dt_dynamic_mode)
-    o.piston_nharm = len(p.piston_a)
+    segs = max(int(getattr(p, "piston_segments", 1)), 1)
+    o.piston_segments = segs
+    o.piston_seg_zmin = float(getattr(p, "piston_seg_zmin", 0.0))
+    o.piston_seg_zmax = float(getattr(p, "piston_seg_zmax", 0.0))
+    o.piston_nharm = len(p.piston_a) // segs
     for k, (a, b) in enumerate(zip(p.piston_a, p.piston_b)):
         o.piston_a[k] = a
         o.piston_b[k] = b

@@ -481,12 +481,16 @@ void update_acceleration_t(const oracle_params& P, const int32_t* itype, const d
 // WaveMaker.cpp:34-82 (first order), cl/TimeIntegration.cl:196-245 (second
 // order coefficient as the product e2_1 * e2_2, matching the documentation),
 // plus the Fourier-series piston x(t) = sum a_k sin(k w t) + b_k cos(k w t).
-double piston_velocity(const oracle_params& P, double time) {
+// A segmented Fourier piston (3D) moves paddle `seg` with its own harmonics
+// (paddle-major coefficients).
+double piston_velocity(const oracle_params& P, double time, int seg = 0) {
     if (P.nwm == 6) {
+        const double* a = P.piston_a + seg * P.piston_nharm;
+        const double* b = P.piston_b + seg * P.piston_nharm;
         double v = 0;
         for (int k = 1; k <= P.piston_nharm; ++k) {
             double kw = k * P.piston_omega;
-            v += kw * (P.piston_a[k - 1] * std::cos(kw * time) - P.piston_b[k - 1] * std::sin(kw * time));
+            v += kw * (a[k - 1] * std::cos(kw * time) - b[k - 1] * std::sin(kw * time));
         }
         return v;
     }
@@ -577,7 +581,14 @@ static int32_t run_steps(oracle_ctx* c, uint32_t nsteps) {
                 // direction gives v = 0; the solitary piston always moves along x
                 double vx = P.nwm == 5 ? solitary_velocity(P, c->time) : piston_velocity(P, c->time);
                 int axis = P.nwm == 5 ? 0 : P.nwm_direction - 1;
+                const int segs = P.piston_segments > 1 && D == 3 && P.nwm == 6 ? P.piston_segments : 1;
+                const double seg_w = segs > 1 ? segs / (P.piston_seg_zmax - P.piston_seg_zmin) : 0.0;
                 for (uint32_t i = P.nwm_particles_start; i < P.nwm_particles_end; ++i) {
+                    if (segs > 1) {  // paddle of the particle: floor((z - zmin) * segs / (zmax - zmin)), clamped
+                        const double u = std::floor((c->r[D * i + D - 1] - P.piston_seg_zmin) * seg_w);
+                        const int g = u < 0.0 ? 0 : (u >= segs ? segs - 1 : (int)u);
+                        vx = piston_velocity(P, c->time, g);
+                    }
                     for (int k = 0; k < D; ++k) c->v[D * i + k] = 0;
                     if (axis >= 0 && axis < D) {
                         c->v[D * i + axis] = vx;

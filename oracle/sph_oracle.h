@@ -98,6 +98,8 @@ typedef struct oracle_params {
     double piston_omega;
     double piston_a[ORACLE_MAX_HARMONICS];
     double piston_b[ORACLE_MAX_HARMONICS];
+    int32_t piston_segments;     /* > 1: paddles across z, harmonics paddle-major */
+    double piston_seg_zmin, piston_seg_zmax;
 
     int32_t boundary_motion;
     int32_t dt_dynamic_mode;

@@ -38,8 +38,10 @@ class DifferentiableRun:
         self.sim.set_piston(self.omega, a, b)
         return self.sim.adjoint_forward(self.nsteps, self.every, self.spec)
 
-    def backward(self, eta_bar: np.ndarray, nharm: int) -> dict:
-        return self.sim.adjoint_backward(eta_bar, nharm)
+    def backward(self, eta_bar: np.ndarray, ncoef: int) -> dict:
+        """Gradients for eta_bar = dL/d eta: the ncoef coefficients of a and
+        of b (every paddle's, paddle-major) and the initial state."""
+        return self.sim.adjoint_backward(eta_bar, ncoef)
 
 
 def _torch():
@@ -63,13 +65,13 @@ class PistonGauges:
                 def forward(ctx, a_t, b_t, run_obj):
                     eta = run_obj.forward(a_t.detach().double().cpu().numpy(), b_t.detach().double().cpu().numpy())
                     ctx.run = run_obj
-                    ctx.nharm = a_t.numel()
+                    ctx.ncoef = a_t.numel()
                     ctx.dtype, ctx.device = a_t.dtype, a_t.device
                     return torch.from_numpy(eta).to(dtype=torch.float64, device=a_t.device)
 
                 @staticmethod
                 def backward(ctx, eta_bar):
-                    g = ctx.run.backward(eta_bar.detach().double().cpu().numpy(), ctx.nharm)
+                    g = ctx.run.backward(eta_bar.detach().double().cpu().numpy(), ctx.ncoef)
                     ga = torch.from_numpy(g["a"]).to(dtype=ctx.dtype, device=ctx.device)
                     gb = torch.from_numpy(g["b"]).to(dtype=ctx.dtype, device=ctx.device)
                     return ga, gb, None

@@ -60,7 +60,10 @@ struct Ctrl {
     uint32_t piston_active;
     uint32_t wall_disappear;
     uint32_t pad1;
+    double piston_vseg[8];  // segmented piston: velocity of each paddle
 };
+
+constexpr int kMaxSegments = 8;  // paddles of a segmented piston
 
 // Everything the kernels need, by value (kernel arguments live in SGPRs).
 struct StepParams {
@@ -106,8 +109,10 @@ struct StepParams {
     double nwm_time_start;
     double nwm_freq, nwm_delta, nwm_S0, nwm_e2;
     double sol_k, sol_c, sol_Tf, sol_H, sol_h;
-    int32_t piston_nharm;
+    int32_t piston_nharm;   // harmonics per paddle
+    int32_t piston_segments;  // paddles across z (1: one piston); coefficients paddle-major
     double piston_omega;
+    double piston_seg_z0, piston_seg_inv_w;  // paddle of a particle: floor((z - z0) * inv_w)
     double piston_a[32];
     double piston_b[32];
     // launch geometry

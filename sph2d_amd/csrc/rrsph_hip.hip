@@ -19,6 +19,7 @@
 #include "radix_sort.hpp"
 #include "resort.hpp"
 #include "adjoint.hpp"
+#include "adjoint3.hpp"
 #include "step_kernels.hpp"
 #include "step3d.hpp"
 #include "pair_tile.hpp"
@@ -82,7 +83,7 @@ double host_w(const rrsph_params_c& p, double dist, int skf) {
 struct AdjTape {
     bool valid = false;
     uint32_t nsteps = 0, every = 0;
-    GaugeSpec spec{};
+    GaugeSpec3 spec{};
     size_t blob = 0;                     // bytes of one saved state
     std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
     void* seg = nullptr;                 // states of one replayed segment (every blobs)
@@ -91,7 +92,7 @@ struct AdjTape {
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
     double *eta_bar = nullptr, *partials = nullptr, *piston_sum = nullptr;
     double *lam[2] = {nullptr, nullptr}, *direct = nullptr, *drho_adj = nullptr;
-    double4* out_adj = nullptr;
+    double4* out_adj = nullptr;          // 3D: out_a (n) then out_x (n)
     uint32_t* flag = nullptr;
 };
 
@@ -535,9 +536,17 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
         P.sol_c = std::sqrt(p.g * (Hs + hd));
         P.sol_Tf = 2 / (P.sol_k * P.sol_c) * (3.8 + Hs / hd);
     }
-    if (p.piston_nharm < 0 || p.piston_nharm > 32)
-        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: piston_nharm must be in [0, 32]");
+    const int32_t segs = p.piston_segments > 1 ? p.piston_segments : 1;
+    if (p.piston_nharm < 0 || segs > kMaxSegments || (int64_t)p.piston_nharm * segs > 32)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT,
+                    "RRSPH_HIP: piston_nharm * piston_segments must be at most 32, piston_segments at most 8");
+    if (segs > 1 && (!d3 || p.nwm != 6 || !(p.piston_seg_zmax > p.piston_seg_zmin)))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT,
+                    "RRSPH_HIP: a segmented piston needs dim 3, nwm 6 and piston_seg_zmax > piston_seg_zmin");
     P.piston_nharm = p.piston_nharm;
+    P.piston_segments = segs;
+    P.piston_seg_z0 = segs > 1 ? p.piston_seg_zmin : 0.0;
+    P.piston_seg_inv_w = segs > 1 ? segs / (p.piston_seg_zmax - p.piston_seg_zmin) : 0.0;
     P.piston_omega = p.piston_omega;
     for (int k = 0; k < 32; ++k) {
         P.piston_a[k] = p.piston_a[k];
@@ -1143,26 +1152,32 @@ rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, u
 }
 
 // ---------------------------------------------------------------- differentiation
-static rrsph_status to_gauge_spec(rrsph_ctx* ctx, const rrsph_gauge_spec* in, GaugeSpec& G) {
+static rrsph_status to_gauge_spec(rrsph_ctx* ctx, const rrsph_gauge_spec* in, GaugeSpec3& G) {
     if (!in || in->n > kMaxGauges || !(in->radius > 0) || !(in->tau > 0))
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad gauge spec");
-    G = GaugeSpec{};
-    G.n = in->n;
-    G.ptype = in->ptype;
-    G.radius = in->radius;
-    G.tau = in->tau;
-    G.y_ref = in->y_ref;
-    for (uint32_t g = 0; g < in->n; ++g) G.x[g] = in->x[g];
+    G = GaugeSpec3{};
+    G.g.n = in->n;
+    G.g.ptype = in->ptype;
+    G.g.radius = in->radius;
+    G.g.tau = in->tau;
+    G.g.y_ref = in->y_ref;
+    for (uint32_t g = 0; g < in->n; ++g) {
+        G.g.x[g] = in->x[g];
+        G.z[g] = in->z[g];
+    }
     return RRSPH_OK;
 }
 
 // smooth gauges of state A into eta/zs/ws (device, kMaxGauges entries each)
-static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec& G, double* partials, double* eta,
+static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, double* partials, double* eta,
                                           double* zs, double* ws) {
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    smooth_gauge_partials<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, G,
-                                                           partials);
-    smooth_gauge_final<<<1, kBlock, 0, ctx->stream>>>(nb, G, partials, eta, zs, ws);
+    if (ctx->P.dim == 3)
+        smooth_gauge_partials3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, G, partials);
+    else
+        smooth_gauge_partials<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, G.g,
+                                                               partials);
+    smooth_gauge_final<<<1, kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws);
     HIP_TRY(hipGetLastError());
     return RRSPH_OK;
 }
@@ -1170,7 +1185,6 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec& G, do
 static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
     const StepParams& P = ctx->P;
     if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint needs an FP64 context");
-    if (P.dim == 3) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint is 2D");
     if (!ctx->fast_pair || P.density_treatment != 1)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: the adjoint needs continuity density, cubic kernels, approximation 2 and no "
@@ -1183,28 +1197,35 @@ static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
 }
 
 // one saved state: posA | vaA | adA | oidA | ctrl
+// bytes per particle of the FP64 position array (2D double2, 3D double4)
+static size_t pos_bytes(const rrsph_ctx* ctx) { return ctx->P.dim == 3 ? 32 : 16; }
+
 static rrsph_status save_state(rrsph_ctx* ctx, void* dst) {
-    const size_t n = ctx->P.n;
+    const size_t n = ctx->P.n, pb = pos_bytes(ctx);
     char* d = (char*)dst;
-    HIP_TRY(hipMemcpyAsync(d, ctx->posA, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + 16 * n, ctx->vaA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + 48 * n, ctx->adA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + 80 * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + 84 * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d, ctx->posA, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + pb * n, ctx->vaA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + 32) * n, ctx->adA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + 64) * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + 68) * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
     return RRSPH_OK;
 }
 
 static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
-    const size_t n = ctx->P.n;
+    const size_t n = ctx->P.n, pb = pos_bytes(ctx);
     const char* d = (const char*)src;
-    HIP_TRY(hipMemcpyAsync(ctx->posA, d, 16 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + 16 * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->adA, d + 48 * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + 80 * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + 84 * n, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->posA, d, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + pb * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->adA, d + (pb + 32) * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + (pb + 64) * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + (pb + 68) * n, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
     // the next step's cell keys from the restored positions, and a full sort
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    state_keys_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA, ctx->key_next);
+    if (ctx->P.dim == 3)
+        state_keys3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->key_next);
+    else
+        state_keys_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
+                                                          ctx->key_next);
     HIP_TRY(hipGetLastError());
     ctx->need_full = true;
     return RRSPH_OK;
@@ -1266,9 +1287,9 @@ static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1
 
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
     if (!ctx || !ctx->uploaded || !eta) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
-    if (ctx->fp32 || ctx->P.dim == 3)
-        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: smooth gauges need a 2D FP64 context");
-    GaugeSpec G;
+    if (ctx->fp32 && ctx->P.dim == 2)
+        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: 2D smooth gauges need an FP64 context");
+    GaugeSpec3 G;
     rrsph_status st = to_gauge_spec(ctx, spec, G);
     if (st != RRSPH_OK) return st;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
@@ -1277,7 +1298,7 @@ rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spe
     double* out = buf + 2 * kMaxGauges * (size_t)nb;
     st = enqueue_smooth_gauges(ctx, G, buf, out, out + kMaxGauges, out + 2 * kMaxGauges);
     if (st == RRSPH_OK) {
-        HIP_TRY(hipMemcpyAsync(eta, out, G.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(eta, out, G.g.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
     HIP_TRY(hipFree(buf));
@@ -1285,14 +1306,16 @@ rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spe
 }
 
 rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b) {
-    if (!ctx || nharm > RRSPH_MAX_HARMONICS || (nharm && (!a || !b)))
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    const uint32_t total = nharm * (uint32_t)ctx->P.piston_segments;  // paddle-major
+    if (nharm > RRSPH_MAX_HARMONICS || total > RRSPH_MAX_HARMONICS || (nharm && (!a || !b)))
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad piston coefficients");
     ctx->prm.piston_omega = omega;
     ctx->prm.piston_nharm = (int32_t)nharm;
     ctx->P.piston_omega = omega;
     ctx->P.piston_nharm = (int32_t)nharm;
     for (uint32_t k = 0; k < RRSPH_MAX_HARMONICS; ++k) {
-        const double ak = k < nharm ? a[k] : 0.0, bk = k < nharm ? b[k] : 0.0;
+        const double ak = k < total ? a[k] : 0.0, bk = k < total ? b[k] : 0.0;
         ctx->prm.piston_a[k] = ak;
         ctx->prm.piston_b[k] = bk;
         ctx->P.piston_a[k] = ak;
@@ -1309,17 +1332,18 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     rrsph_status st = adjoint_supported(ctx);
     if (st != RRSPH_OK) return st;
-    GaugeSpec G;
+    GaugeSpec3 G;
     st = to_gauge_spec(ctx, spec, G);
     if (st != RRSPH_OK) return st;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     free_tape(ctx->adj);
     AdjTape& T = ctx->adj;
     const size_t n = ctx->P.n;
+    const bool d3 = ctx->P.dim == 3;
     T.nsteps = nsteps;
     T.every = std::min(checkpoint_every, nsteps);
     T.spec = G;
-    T.blob = ((84 * n + sizeof(Ctrl)) + 255) / 256 * 256;
+    T.blob = (((pos_bytes(ctx) + 68) * n + sizeof(Ctrl)) + 255) / 256 * 256;
     const uint32_t nck = (nsteps + T.every - 1) / T.every;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     hipError_t e = hipSuccess;
@@ -1331,12 +1355,12 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
     T.ws = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.eta_bar = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.partials = dalloc<double>(2 * kMaxGauges * (size_t)nb, e);
-    T.piston_sum = dalloc<double>(nsteps, e);
-    T.lam[0] = dalloc<double>(8 * n, e);
-    T.lam[1] = dalloc<double>(8 * n, e);
-    T.direct = dalloc<double>(5 * n, e);
+    T.piston_sum = dalloc<double>((size_t)nsteps * kMaxSegments, e);
+    T.lam[0] = dalloc<double>((d3 ? kLam3 : 8) * n, e);
+    T.lam[1] = dalloc<double>((d3 ? kLam3 : 8) * n, e);
+    T.direct = dalloc<double>((d3 ? kDirect3 : 5) * n, e);
     T.drho_adj = dalloc<double>(n, e);
-    T.out_adj = dalloc<double4>(n, e);
+    T.out_adj = dalloc<double4>((d3 ? 2 : 1) * n, e);
     T.flag = dalloc<uint32_t>(1, e);
     if (e != hipSuccess) {
         free_tape(ctx->adj);
@@ -1359,9 +1383,22 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
     HIP_TRY(hipMemcpyAsync(T.hrec.data(), T.rec, nsteps * sizeof(AdjStep), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < nsteps; ++k)
-        for (uint32_t g = 0; g < G.n; ++g) eta[(size_t)k * G.n + g] = h[(size_t)k * kMaxGauges + g];
+        for (uint32_t g = 0; g < G.g.n; ++g) eta[(size_t)k * G.g.n + g] = h[(size_t)k * kMaxGauges + g];
     T.valid = true;
     return RRSPH_OK;
+}
+
+// dL/dx of the loss term of the gauges of state A (lam += ...)
+static void enqueue_gauge_grad(rrsph_ctx* ctx, uint32_t k, double* lam) {
+    AdjTape& T = ctx->adj;
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    const size_t o = (size_t)k * kMaxGauges;
+    if (ctx->P.dim == 3)
+        smooth_gauge_grad3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, ctx->oidA, T.spec,
+                                                           T.eta_bar + o, T.zs + o, T.ws + o, lam);
+    else
+        smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA,
+                                                          T.spec.g, T.eta_bar + o, T.zs + o, T.ws + o, lam);
 }
 
 rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
@@ -1370,8 +1407,10 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
     AdjTape& T = ctx->adj;
     if (!T.valid) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: run rrsph_hip_adjoint_forward first");
     const StepParams& P = ctx->P;
+    const bool d3 = P.dim == 3;
     const size_t n = P.n;
-    const uint32_t N = T.nsteps, G = T.spec.n;
+    const uint32_t lw = d3 ? kLam3 : 8;  // doubles of adjoint state per particle
+    const uint32_t N = T.nsteps, G = T.spec.g.n;
     const uint32_t nb = grid_blocks(P.n, kBlock);
     {
         std::vector<double> h((size_t)N * kMaxGauges, 0.0);
@@ -1380,17 +1419,14 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         HIP_TRY(hipMemcpyAsync(T.eta_bar, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
-    HIP_TRY(hipMemsetAsync(T.piston_sum, 0, N * sizeof(double), ctx->stream));
-    HIP_TRY(hipMemsetAsync(T.lam[0], 0, 8 * n * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(T.piston_sum, 0, (size_t)N * kMaxSegments * sizeof(double), ctx->stream));
+    HIP_TRY(hipMemsetAsync(T.lam[0], 0, lw * n * sizeof(double), ctx->stream));
     HIP_TRY(hipMemsetAsync(T.flag, 0, sizeof(uint32_t), ctx->stream));
     const uint32_t nck = (N + T.every - 1) / T.every;
     // the loss term of the final state
     rrsph_status st = restore_state(ctx, T.ck[nck]);
     if (st != RRSPH_OK) return st;
-    smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA, T.spec,
-                                                      T.eta_bar + (size_t)(N - 1) * kMaxGauges,
-                                                      T.zs + (size_t)(N - 1) * kMaxGauges,
-                                                      T.ws + (size_t)(N - 1) * kMaxGauges, T.lam[0]);
+    enqueue_gauge_grad(ctx, N - 1, T.lam[0]);
     HIP_TRY(hipGetLastError());
     int cur = 0;
     for (int seg = (int)nck - 1; seg >= 0; --seg) {
@@ -1406,50 +1442,57 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
             // sort and reorder state k exactly as its step did (predict with its dt)
             adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
             enqueue_util_sort<double>(ctx);
-            StateB<double> B = stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch);
             const AdjStep& S = T.hrec[k];
-            adj_prep<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.drho_adj, T.direct,
-                                                     T.piston_sum + k);
-            adj_pair<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
-                                                     T.direct, T.lam[1 - cur], T.flag);
-            if (k >= 1)  // loss term of state k (the gauges after step k - 1)
-                smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(
-                    P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA, T.spec,
-                    T.eta_bar + (size_t)(k - 1) * kMaxGauges, T.zs + (size_t)(k - 1) * kMaxGauges,
-                    T.ws + (size_t)(k - 1) * kMaxGauges, T.lam[1 - cur]);
+            double* psum = T.piston_sum + (size_t)k * kMaxSegments;
+            if (d3) {
+                StateB3<double> B = stateB3<double>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
+                adj_prep3<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.out_adj + n, T.direct,
+                                                          psum);
+                adj_pair3<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj,
+                                                          T.out_adj + n, T.direct, T.lam[1 - cur], T.flag);
+            } else {
+                StateB<double> B = stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch);
+                adj_prep<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.drho_adj, T.direct,
+                                                         psum);
+                adj_pair<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
+                                                         T.direct, T.lam[1 - cur], T.flag);
+            }
+            if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur]);  // loss term of state k
             HIP_TRY(hipGetLastError());
             cur = 1 - cur;
         }
     }
-    // piston coefficients: dV/da_k = k w cos(k w t), dV/db_k = -k w sin(k w t)
-    std::vector<double> ps(N);
-    HIP_TRY(hipMemcpyAsync(ps.data(), T.piston_sum, N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<double> lam(8 * n);
-    HIP_TRY(hipMemcpyAsync(lam.data(), T.lam[cur], 8 * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // piston coefficients of paddle g: dV_g/da_{g,k} = k w cos(k w t), dV_g/db_{g,k} = -k w sin(k w t)
+    std::vector<double> ps((size_t)N * kMaxSegments);
+    HIP_TRY(hipMemcpyAsync(ps.data(), T.piston_sum, ps.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    std::vector<double> lam(lw * n);
+    HIP_TRY(hipMemcpyAsync(lam.data(), T.lam[cur], lam.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     uint32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, T.flag, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const int nh = P.nwm == 6 ? P.piston_nharm : 0;
-    for (int k = 0; k < (int)RRSPH_MAX_HARMONICS && k < std::max(nh, 0); ++k) {
-        double ga = 0, gb = 0;
-        const double kw = (k + 1) * P.piston_omega;
-        for (uint32_t s = 0; s < N; ++s) {
-            ga += ps[s] * kw * std::cos(kw * T.hrec[s].t);
-            gb -= ps[s] * kw * std::sin(kw * T.hrec[s].t);
+    const int nh = P.nwm == 6 ? std::max(P.piston_nharm, 0) : 0;
+    for (int g = 0; g < P.piston_segments; ++g) {
+        for (int k = 0; k < nh; ++k) {
+            double ga = 0, gb = 0;
+            const double kw = (k + 1) * P.piston_omega;
+            for (uint32_t s = 0; s < N; ++s) {
+                const double v = ps[(size_t)s * kMaxSegments + g];
+                ga += v * kw * std::cos(kw * T.hrec[s].t);
+                gb -= v * kw * std::sin(kw * T.hrec[s].t);
+            }
+            if (grad_a) grad_a[g * nh + k] = ga;
+            if (grad_b) grad_b[g * nh + k] = gb;
         }
-        if (grad_a) grad_a[k] = ga;
-        if (grad_b) grad_b[k] = gb;
     }
+    const uint32_t D = d3 ? 3 : 2;
+    const uint32_t vo = d3 ? 3 : 2, ro = d3 ? 6 : 4;  // offsets of v and rho in lam
     for (size_t i = 0; i < n; ++i) {
-        if (grad_r0) {
-            grad_r0[2 * i] = lam[8 * i];
-            grad_r0[2 * i + 1] = lam[8 * i + 1];
+        for (uint32_t c = 0; c < D; ++c) {
+            if (grad_r0) grad_r0[D * i + c] = lam[lw * i + c];
+            if (grad_v0) grad_v0[D * i + c] = lam[lw * i + vo + c];
         }
-        if (grad_v0) {
-            grad_v0[2 * i] = lam[8 * i + 2];
-            grad_v0[2 * i + 1] = lam[8 * i + 3];
-        }
-        if (grad_rho0) grad_rho0[i] = lam[8 * i + 4];
+        if (grad_rho0) grad_rho0[i] = lam[lw * i + ro];
     }
     // leave the context as the forward left it
     st = restore_state_sync(ctx, T.ck[nck]);

@@ -281,9 +281,10 @@ __device__ __forceinline__ void integrate_one3(const StepParams& P, const Ctrl& 
         }
     } else if (P.boundary_motion && itype < 0) {
         if (c.piston_active && oid >= P.nwm_start && oid < P.nwm_end && oid >= P.nfluid) {
-            vx = P.nwm_axis == 0 ? (R)c.piston_v : R(0);
-            vy = P.nwm_axis == 1 ? (R)c.piston_v : R(0);
-            vz = P.nwm_axis == 2 ? (R)c.piston_v : R(0);
+            const double pv = P.piston_segments > 1 ? c.piston_vseg[piston_segment(P, pos.z)] : c.piston_v;
+            vx = P.nwm_axis == 0 ? (R)pv : R(0);
+            vy = P.nwm_axis == 1 ? (R)pv : R(0);
+            vz = P.nwm_axis == 2 ? (R)pv : R(0);
         }
         pos.x += (double)vx * dt;
         pos.y += (double)vy * dt;

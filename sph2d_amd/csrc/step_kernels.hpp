@@ -64,12 +64,15 @@ __device__ __forceinline__ uint32_t split2(uint32_t x) {
 }
 
 // ------------------------------------------------------------------ prologue
-__device__ double piston_velocity(const StepParams& P, double t) {
+// Velocity of the wave maker at time t (paddle `seg` of a segmented piston).
+__device__ double piston_velocity(const StepParams& P, double t, int seg = 0) {
     if (P.nwm == 6) {
+        const double* a = P.piston_a + seg * P.piston_nharm;
+        const double* b = P.piston_b + seg * P.piston_nharm;
         double v = 0;
         for (int k = 1; k <= P.piston_nharm; ++k) {
             const double kw = k * P.piston_omega;
-            v += kw * (P.piston_a[k - 1] * cos(kw * t) - P.piston_b[k - 1] * sin(kw * t));
+            v += kw * (a[k - 1] * cos(kw * t) - b[k - 1] * sin(kw * t));
         }
         return v;
     }
@@ -84,6 +87,12 @@ __device__ double piston_velocity(const StepParams& P, double t) {
     double vx = 0.5 * P.nwm_S0 * P.nwm_freq * cos(P.nwm_freq * t + P.nwm_delta);
     if (P.nwm == 3) vx += 2 * P.nwm_freq * P.nwm_e2 * cos(2 * P.nwm_freq * t + 2 * P.nwm_delta);
     return vx;
+}
+
+// Paddle of a segmented piston that a wave-maker particle at z belongs to.
+__device__ __forceinline__ int piston_segment(const StepParams& P, double z) {
+    const double u = floor((z - P.piston_seg_z0) * P.piston_seg_inv_w);
+    return u < 0.0 ? 0 : (u >= (double)P.piston_segments ? P.piston_segments - 1 : (int)u);
 }
 
 __device__ __forceinline__ double dbits(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -177,6 +186,8 @@ __global__ void step_prologue(StepParams P, Ctrl* ctrl) {
     const bool nwm_on = P.boundary_motion && P.nwm != 0 && c.time >= P.nwm_time_start;
     c.piston_active = nwm_on && P.nwm != 4 && P.nwm_axis >= 0;
     c.piston_v = c.piston_active ? piston_velocity(P, c.time) : 0.0;
+    for (int g = 0; g < kMaxSegments; ++g)
+        c.piston_vseg[g] = c.piston_active && g < P.piston_segments ? piston_velocity(P, c.time, g) : 0.0;
     c.wall_disappear = nwm_on && P.nwm == 4 && !c.nwm_done;
     if (c.wall_disappear) c.nwm_done = 1;
     c.amax_bits = 0;

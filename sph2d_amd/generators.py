@@ -198,9 +198,12 @@ def piston_tank(delta: float = 0.0075, length: float = 20.0, depth: float = 0.5,
                 fourier: Tuple[Tuple[float, ...], Tuple[float, ...], float] = None) -> Layout:
     """2D piston wave flume: fluid length x depth at rest with hydrostatic density,
     a vertical piston wall on the left moving as x(t) = A sin(2 pi t / T) (or a
-    Fourier series), 3-layer LJ walls, continuity density, dynamic CFL dt."""
+    Fourier series), 3-layer LJ walls, continuity density, dynamic CFL dt.  The
+    fluid starts 2 delta from the walls, where the reference's piston
+    generator puts it (tools/Scripts/Piston2DGenerator.py:108), inside the 2h
+    interaction range of the piston and the floor."""
     nx, ny = int(round(length / delta)), int(round(depth / delta))
-    gap = 2.0 * hsml_coef * delta  # Lennard-Jones zero-force distance r0 = 2h
+    gap = 2.0 * delta
     xs = gap + delta * np.arange(nx)
     ys = gap + delta * np.arange(ny)
     X, Y = np.meshgrid(xs, ys, indexing="ij")
@@ -263,6 +266,75 @@ def adjoint_tank(delta: float = 0.005, length: float = 8.0, depth: float = 0.4, 
     prm.dt_dynamic_mode = 0
     prm.dt = cfl * prm.hsml / prm.eos_c
     return lay
+
+
+def piston_basin_3d(delta: float = 0.01, length: float = 6.0, width: float = 2.0, depth: float = 0.5,
+                    wall_height: float = 1.0, layers: int = 3, segments: int = 8, nharm: int = 4,
+                    period: float = 1.6, sigma: float = 0.01, seed: int = 20240605, hsml_coef: float = 1.2,
+                    cfl: float = 0.2, max_neighbours: int = 128) -> Layout:
+    """3D differentiable piston wave basin (y up): fluid length x depth x width
+    at rest with hydrostatic density; the wall at x = 0 is a piston split into
+    `segments` equal paddles across z, each moving as its own Fourier series
+    of `nharm` harmonics at w = 2 pi / period (coefficients N(0, sigma^2) from
+    `seed`, paddle-major); 3-layer LJ walls on the floor, the two sides and
+    the far end; continuity density; fixed dt = cfl h / c.  The fluid starts
+    2 delta from every wall, as the reference's piston generator places it
+    (tools/Scripts/Piston2DGenerator.py:108), inside the 2h interaction
+    range, so every paddle acts on the fluid from the first step."""
+    gap = 2.0 * delta
+    nx, ny = int(round(length / delta)), int(round(depth / delta))
+    nz = int(round((width - 2 * gap) / delta)) + 1
+    xs = gap + delta * np.arange(nx)
+    ys = gap + delta * np.arange(ny)
+    zs = gap + delta * np.arange(nz)
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    fluid = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    L_end = xs[-1] + gap
+    W = zs[-1] + gap
+    surface = ys[-1] + 0.5 * delta
+    x0, x1 = -(layers + 1) * delta, L_end + (layers + 1) * delta
+    piston = _wall_plane(0, 0.0, -1, ((delta, wall_height), (delta, W - delta)), delta, layers)
+    floor = _wall_plane(1, 0.0, -1, ((x0, x1), (0.0, W)), delta, layers)
+    side0 = _wall_plane(2, 0.0, -1, ((x0, x1), (delta, wall_height)), delta, layers)
+    side1 = _wall_plane(2, W, +1, ((x0, x1), (delta, wall_height)), delta, layers)
+    end = _wall_plane(0, L_end, +1, ((delta, wall_height), (delta, W - delta)), delta, layers)
+    nfluid = len(fluid)
+    r = np.concatenate([fluid, piston, floor, side0, side1, end])
+    n = len(r)
+    margin = (layers + 3) * delta
+    prm = _base_params(delta, hsml_coef, n, nfluid, (x0 - margin - 0.2, x1 + margin, -margin, wall_height + 0.6),
+                       depth)
+    prm.dim = 3
+    prm.z_mingeom, prm.z_maxgeom = -margin - layers * delta, W + margin + layers * delta
+    prm.max_neighbours = max_neighbours
+    prm.density_treatment = P.DENSITY_CONTINUITY
+    prm.boundary_motion = True
+    prm.nwm = P.NWM_METHOD_FOURIER
+    prm.nwm_direction = P.NWM_DIRECTION_AXIS_X
+    prm.nwm_particles_start = nfluid
+    prm.nwm_particles_end = nfluid + len(piston)
+    prm.nwm_time_start = 0.0
+    rng = np.random.default_rng(seed)
+    prm.piston_a = tuple(rng.normal(0.0, sigma, segments * nharm))
+    prm.piston_b = tuple(rng.normal(0.0, sigma, segments * nharm))
+    prm.piston_omega = 2 * math.pi / period
+    # paddle boundaries midway between the rows of wall particles (z on a
+    # delta / 2 lattice), so that no particle sits on a boundary
+    half = 0.5 * delta
+    w = max(round(W / segments / half), 1) * half
+    z0 = 0.5 * half + round(0.5 * (W - segments * w) / half) * half
+    prm.piston_segments = segments
+    prm.piston_seg_zmin, prm.piston_seg_zmax = z0, z0 + segments * w
+    prm.dt_correction_method = P.DT_CORRECTION_CONST_VALUE
+    prm.dt_dynamic_mode = 0
+    P.fill_in_computing_params(prm)
+    P.post_fill_in_model_params(prm)
+    prm.dt = cfl * prm.hsml / prm.eos_c
+    prm.simulation_time = 1e9
+    v = np.zeros_like(r)
+    rho = hydrostatic_rho(r[:, 1], surface, prm.rho0, prm.eos_c)
+    itype = np.concatenate([np.full(nfluid, P.TYPE_WATER), np.full(n - nfluid, P.TYPE_BOUNDARY)]).astype(np.int32)
+    return Layout(prm, r, v, rho, itype)
 
 
 def c1_1m_delta() -> float:

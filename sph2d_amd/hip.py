@@ -76,6 +76,8 @@ class RrsphParams(ctypes.Structure):
         ("piston_a", ctypes.c_double * MAX_HARMONICS), ("piston_b", ctypes.c_double * MAX_HARMONICS),
         ("boundary_motion", ctypes.c_int32), ("dt_dynamic_mode", ctypes.c_int32),
         ("precision", ctypes.c_int32), ("start_time", ctypes.c_double),
+        ("piston_segments", ctypes.c_int32), ("piston_seg_zmin", ctypes.c_double),
+        ("piston_seg_zmax", ctypes.c_double),
     ]
 
 
@@ -93,20 +95,26 @@ MAX_GAUGES = 64
 
 
 class GaugeSpec(ctypes.Structure):
-    """rrsph_gauge_spec: smooth gauge surrogate (window radius, softness tau)."""
+    """rrsph_gauge_spec: smooth gauge surrogate (window radius, softness tau);
+    gauges at x (2D) or at (x, z) (3D)."""
     _fields_ = [("n", ctypes.c_uint32), ("ptype", ctypes.c_int32), ("radius", ctypes.c_double),
-                ("tau", ctypes.c_double), ("y_ref", ctypes.c_double), ("x", ctypes.c_double * MAX_GAUGES)]
+                ("tau", ctypes.c_double), ("y_ref", ctypes.c_double), ("x", ctypes.c_double * MAX_GAUGES),
+                ("z", ctypes.c_double * MAX_GAUGES)]
 
 
-def gauge_spec(xs: Sequence[float], radius: float, tau: float, y_ref: float, ptype: int = 2) -> GaugeSpec:
+def gauge_spec(xs: Sequence[float], radius: float, tau: float, y_ref: float, ptype: int = 2,
+               zs: Optional[Sequence[float]] = None) -> GaugeSpec:
     if len(xs) > MAX_GAUGES:
         raise ValueError(f"at most {MAX_GAUGES} gauges")
+    if zs is not None and len(zs) != len(xs):
+        raise ValueError("xs and zs need the same length")
     g = GaugeSpec()
     g.n = len(xs)
     g.ptype = int(ptype)
     g.radius, g.tau, g.y_ref = float(radius), float(tau), float(y_ref)
     for i, x in enumerate(xs):
         g.x[i] = float(x)
+        g.z[i] = float(zs[i]) if zs is not None else 0.0
     return g
 
 
@@ -175,7 +183,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 def make_c_params(prm: "P.ExperimentParams", precision: int = FP64) -> RrsphParams:
     c = RrsphParams()
     c.struct_size = ctypes.sizeof(RrsphParams)
-    skip = {"struct_size", "eos_c_sqr", "eos_c", "piston_a", "piston_b", "piston_nharm", "precision", "start_time"}
+    skip = {"struct_size", "eos_c_sqr", "eos_c", "piston_a", "piston_b", "piston_nharm", "precision", "start_time",
+            "piston_segments", "piston_seg_zmin", "piston_seg_zmax"}
     for name, _t in RrsphParams._fields_:
         if name in skip:
             continue
@@ -184,9 +193,13 @@ def make_c_params(prm: "P.ExperimentParams", precision: int = FP64) -> RrsphPara
     c.eos_c = prm.eos_c
     c.boundary_motion = int(prm.boundary_motion)
     c.dt_dynamic_mode = int(prm.dt_dynamic_mode)
-    c.piston_nharm = len(prm.piston_a)
-    if c.piston_nharm > MAX_HARMONICS:
-        raise ValueError("at most 32 piston harmonics")
+    segs = max(int(getattr(prm, "piston_segments", 1)), 1)
+    if len(prm.piston_a) > MAX_HARMONICS or len(prm.piston_a) % segs:
+        raise ValueError("at most 32 piston coefficients per array, a multiple of piston_segments")
+    c.piston_nharm = len(prm.piston_a) // segs
+    c.piston_segments = segs
+    c.piston_seg_zmin = float(getattr(prm, "piston_seg_zmin", 0.0))
+    c.piston_seg_zmax = float(getattr(prm, "piston_seg_zmax", 0.0))
     for k, (a, b) in enumerate(zip(prm.piston_a, prm.piston_b)):
         c.piston_a[k] = a
         c.piston_b[k] = b
@@ -309,7 +322,10 @@ class HipStep:
         b = np.ascontiguousarray(b, dtype=np.float64)
         if a.shape != b.shape:
             raise ValueError("a and b need the same length")
-        self._check(self.L.rrsph_hip_set_piston(self.ctx, float(omega), len(a), _dp(a), _dp(b)))
+        segs = self.cparams.piston_segments
+        if len(a) % segs:
+            raise ValueError("the coefficients of every paddle: a multiple of piston_segments")
+        self._check(self.L.rrsph_hip_set_piston(self.ctx, float(omega), len(a) // segs, _dp(a), _dp(b)))
 
     def adjoint_forward(self, nsteps: int, every: int, spec: GaugeSpec) -> np.ndarray:
         """Differentiable run from the uploaded state; gauges after each step,
@@ -319,15 +335,17 @@ class HipStep:
                                                      _dp(eta)))
         return eta
 
-    def adjoint_backward(self, eta_bar: np.ndarray, nharm: int) -> Dict[str, np.ndarray]:
+    def adjoint_backward(self, eta_bar: np.ndarray, ncoef: int) -> Dict[str, np.ndarray]:
         """Gradients of the loss whose derivative with respect to the gauges
-        is eta_bar: piston coefficients a, b and the initial state."""
+        is eta_bar: the ncoef piston coefficients of a and of b (every
+        paddle's, paddle-major) and the initial state."""
         eta_bar = np.ascontiguousarray(eta_bar, dtype=np.float64)
-        ga, gb = np.zeros(max(nharm, 1)), np.zeros(max(nharm, 1))
-        gr, gv, grho = np.zeros((self.n, 2)), np.zeros((self.n, 2)), np.zeros(self.n)
+        ga, gb = np.zeros(MAX_HARMONICS), np.zeros(MAX_HARMONICS)
+        d = self.dim
+        gr, gv, grho = np.zeros((self.n, d)), np.zeros((self.n, d)), np.zeros(self.n)
         self._check(self.L.rrsph_hip_adjoint_backward(self.ctx, _dp(eta_bar), _dp(ga), _dp(gb), _dp(gr), _dp(gv),
                                                       _dp(grho)))
-        return {"a": ga[:nharm], "b": gb[:nharm], "r": gr, "v": gv, "rho": grho}
+        return {"a": ga[:ncoef], "b": gb[:ncoef], "r": gr, "v": gv, "rho": grho}
 
     def sort_stats(self) -> Dict[str, int]:
         """Steps sorted incrementally / fully, re-sort fallbacks, particles

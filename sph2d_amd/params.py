@@ -114,9 +114,10 @@ _MODEL_KEYS = [
     ("save_density", "default", True),
 ]
 
-# Keys of this build's extension (Fourier-series piston); they are ignored by
-# the reference loader and optional here.
-_EXTENSION_KEYS = ("piston_omega", "piston_a", "piston_b")
+# Keys of this build's extension (Fourier-series piston, optionally split
+# into paddles across z); they are ignored by the reference loader and
+# optional here.
+_EXTENSION_KEYS = ("piston_omega", "piston_a", "piston_b", "piston_segments", "piston_seg_zmin", "piston_seg_zmax")
 
 
 def _version_tuple(d: Dict[str, Any]):
@@ -317,10 +318,15 @@ class ExperimentParams:
     use_crash_dump: bool = True
     use_dump: bool = False
     dump_time: float = 0.0
-    # extension: Fourier-series piston
+    # extension: Fourier-series piston; with piston_segments > 1 (3D) the
+    # coefficients hold every paddle's harmonics, paddle-major, and the
+    # paddles split [piston_seg_zmin, piston_seg_zmax) into equal parts
     piston_omega: float = 0.0
     piston_a: tuple = ()
     piston_b: tuple = ()
+    piston_segments: int = 1
+    piston_seg_zmin: float = 0.0
+    piston_seg_zmax: float = 0.0
     # switches for RRSPH_CL-only behaviour (described in the oracle header)
     boundary_motion: bool = False
     dt_dynamic_mode: int = 0
@@ -357,6 +363,10 @@ def apply_model_params(params: ExperimentParams, mp: Dict[str, Any]) -> None:
         params.piston_omega = float(mp["piston_omega"])
         params.piston_a = tuple(float(x) for x in mp["piston_a"])
         params.piston_b = tuple(float(x) for x in mp["piston_b"])
+    if mp.get("piston_segments") is not None:
+        params.piston_segments = int(mp["piston_segments"])
+        params.piston_seg_zmin = float(mp["piston_seg_zmin"])
+        params.piston_seg_zmax = float(mp["piston_seg_zmax"])
 
 
 def apply_particle_params(params: ExperimentParams, pp: Dict[str, Any]) -> None:

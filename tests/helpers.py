@@ -95,14 +95,16 @@ def brute_force_sets(r, cut2, itype):
     return out
 
 
-def smooth_gauges_np(r, itype, xs, radius, tau, y_ref, ptype=2):
+def smooth_gauges_np(r, itype, xs, radius, tau, y_ref, ptype=2, zs=None):
     """The smooth gauge surrogate in numpy (the definition in include/rrsph_hip.h):
-    eta = y_ref + tau log(sum w e / sum w), w = (1 - (x - x_g)^2 / R^2)^3, e = exp((y - y_ref) / tau)."""
+    eta = y_ref + tau log(sum w e / sum w), w = (1 - d^2 / R^2)^3, e = exp((y - y_ref) / tau),
+    d = x - x_g (2D) or the distance from (x_g, z_g) in the x-z plane (3D, zs given)."""
     sel = itype == ptype
     x, y = r[sel, 0], r[sel, 1]
     out = np.zeros(len(xs))
     for g, xg in enumerate(xs):
-        u = 1.0 - (x - xg) ** 2 / radius ** 2
+        d2 = (x - xg) ** 2 if zs is None else (x - xg) ** 2 + (r[sel, 2] - zs[g]) ** 2
+        u = 1.0 - d2 / radius ** 2
         w = np.where(u > 0, u ** 3, 0.0)
         ws = w.sum()
         if ws > 0:

@@ -45,9 +45,10 @@ def test_params_struct_layout_matches_c():
 #include <stddef.h>
 #include "rrsph_hip.h"
 int main(void) {
-  printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(rrsph_params_c), offsetof(rrsph_params_c, piston_a),
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(rrsph_params_c), offsetof(rrsph_params_c, piston_a),
          offsetof(rrsph_params_c, precision), offsetof(rrsph_params_c, start_time), sizeof(rrsph_diagnostics),
-         sizeof(rrsph_gauge_spec), offsetof(rrsph_gauge_spec, x));
+         sizeof(rrsph_gauge_spec), offsetof(rrsph_gauge_spec, x), offsetof(rrsph_params_c, piston_seg_zmax),
+         offsetof(rrsph_gauge_spec, z));
   return 0;
 }
 """
@@ -57,8 +58,9 @@ int main(void) {
         open(c, "w").write(src)
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", exe, c], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()
-    size, off_a, off_prec, off_t, dsize, gsize, goff = map(int, out)
-    assert gsize == ctypes.sizeof(H.GaugeSpec) and goff == H.GaugeSpec.x.offset
+    size, off_a, off_prec, off_t, dsize, gsize, goff, off_seg, goffz = map(int, out)
+    assert gsize == ctypes.sizeof(H.GaugeSpec) and goff == H.GaugeSpec.x.offset and goffz == H.GaugeSpec.z.offset
+    assert off_seg == H.RrsphParams.piston_seg_zmax.offset
     assert size == ctypes.sizeof(H.RrsphParams)
     assert off_a == H.RrsphParams.piston_a.offset
     assert off_prec == H.RrsphParams.precision.offset
@@ -101,6 +103,8 @@ def _no_device():
         ({"max_neighbours": 1}, 1, "invalid sizes"),
         ({"x_maxgeom": 1.0e9}, 6, "can't make grid with so many cells"),
         ({"nfluid": 10**7}, 1, "nfluid > ntotal"),
+        ({"piston_segments": 2, "piston_seg_zmax": 1.0}, 1, "a segmented piston needs dim 3"),
+        ({"piston_segments": 9}, 1, "piston_segments at most 8"),
     ],
 )
 def test_create_validates_params_on_host(change, status, text):
