@@ -45,25 +45,26 @@ struct GaugeSpec {
 };
 
 // ------------------------------------------------------------------ smooth gauges
-// Per-block partial sums (sum w e, sum w) of every gauge.
-__global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, const double2* __restrict__ pos,
-                                                                const double4* __restrict__ va, GaugeSpec G,
-                                                                double* __restrict__ partials) {
+// Block partial sums (sum w e, sum w) of every gauge, gauge-major:
+// partials[(2 g) nb + b] = sum w e, partials[(2 g + 1) nb + b] = sum w of
+// block b (nb = gridDim.x).  A block with no particle inside any window
+// writes zeros without the per-gauge reductions (the sums are exactly zero).
+template <class D2>
+__device__ __forceinline__ void gauge_block_sums(const GaugeSpec& G, bool in, double y, D2&& d2, double inv_r2,
+                                                 double* __restrict__ partials) {
     __shared__ double s_z[kBlock / kWave][kMaxGauges], s_w[kBlock / kWave][kMaxGauges];
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t wave = threadIdx.x / kWave;
-    bool in = false;
-    double x = 0, y = 0;
-    if (s < n) {
-        const double2 p = pos[s];
-        in = (int)va[s].w == G.ptype;
-        x = p.x;
-        y = p.y;
+    const uint32_t wave = threadIdx.x / kWave, nb = gridDim.x;
+    bool near = false;
+    for (uint32_t g = 0; g < G.n; ++g) near |= in && d2(g) * inv_r2 < 1.0;
+    if (!__syncthreads_or(near)) {
+        for (uint32_t g = threadIdx.x; g < G.n; g += kBlock) {
+            partials[(size_t)(2 * g) * nb + blockIdx.x] = 0.0;
+            partials[(size_t)(2 * g + 1) * nb + blockIdx.x] = 0.0;
+        }
+        return;
     }
-    const double inv_r2 = 1.0 / (G.radius * G.radius);
     for (uint32_t g = 0; g < G.n; ++g) {
-        const double dx = x - G.x[g];
-        const double u = 1.0 - dx * dx * inv_r2;
+        const double u = 1.0 - d2(g) * inv_r2;
         const double w = in && u > 0.0 ? u * u * u : 0.0;
         double z = w > 0.0 ? w * exp((y - G.y_ref) / G.tau) : 0.0;
         double ww = w;
@@ -84,35 +85,64 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, cons
             z += s_z[w][g];
             ww += s_w[w][g];
         }
-        partials[(size_t)blockIdx.x * 2 * kMaxGauges + g] = z;
-        partials[(size_t)blockIdx.x * 2 * kMaxGauges + kMaxGauges + g] = ww;
+        partials[(size_t)(2 * g) * nb + blockIdx.x] = z;
+        partials[(size_t)(2 * g + 1) * nb + blockIdx.x] = ww;
     }
 }
 
-// One block: wave w reduces gauges w, w + 4, ...: each lane sums a fixed
-// stride of the block partials, then a fixed shuffle tree (deterministic);
+__global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, const double2* __restrict__ pos,
+                                                                const double4* __restrict__ va, GaugeSpec G,
+                                                                double* __restrict__ partials) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    bool in = false;
+    double x = 0, y = 0;
+    if (s < n) {
+        const double2 p = pos[s];
+        in = (int)va[s].w == G.ptype;
+        x = p.x;
+        y = p.y;
+    }
+    auto d2 = [&](uint32_t g) { return (x - G.x[g]) * (x - G.x[g]); };
+    gauge_block_sums(G, in, y, d2, 1.0 / (G.radius * G.radius), partials);
+}
+
+// One block per gauge: each thread sums a fixed stride of the block
+// partials, then a fixed shuffle tree and wave order (deterministic);
 // writes eta[g], Z[g] = sum w e and Wsum[g] = sum w.
 __global__ void __launch_bounds__(kBlock) smooth_gauge_final(uint32_t nblocks, GaugeSpec G,
                                                              const double* __restrict__ partials,
                                                              double* __restrict__ eta, double* __restrict__ zsum,
                                                              double* __restrict__ wsum) {
-    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
-    for (uint32_t g = wave; g < G.n; g += kBlock / kWave) {
-        double z = 0, ww = 0;
-        for (uint32_t b = lane; b < nblocks; b += kWave) {
-            z += partials[(size_t)b * 2 * kMaxGauges + g];
-            ww += partials[(size_t)b * 2 * kMaxGauges + kMaxGauges + g];
-        }
+    __shared__ double s_z[kBlock / kWave], s_w[kBlock / kWave];
+    const uint32_t g = blockIdx.x;
+    if (g >= G.n) return;
+    const double* pz = partials + (size_t)(2 * g) * nblocks;
+    const double* pw = partials + (size_t)(2 * g + 1) * nblocks;
+    double z = 0, ww = 0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kBlock) {
+        z += pz[b];
+        ww += pw[b];
+    }
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            z += __shfl_xor(z, off);
-            ww += __shfl_xor(ww, off);
+    for (int off = 32; off > 0; off >>= 1) {
+        z += __shfl_xor(z, off);
+        ww += __shfl_xor(ww, off);
+    }
+    if (__lane_id() == 0) {
+        s_z[threadIdx.x / kWave] = z;
+        s_w[threadIdx.x / kWave] = ww;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        z = 0;
+        ww = 0;
+        for (uint32_t w = 0; w < kBlock / kWave; ++w) {
+            z += s_z[w];
+            ww += s_w[w];
         }
-        if (lane == 0) {
-            eta[g] = ww > 0.0 ? G.y_ref + G.tau * log(z / ww) : 0.0;
-            zsum[g] = z;
-            wsum[g] = ww;
-        }
+        eta[g] = ww > 0.0 ? G.y_ref + G.tau * log(z / ww) : 0.0;
+        zsum[g] = z;
+        wsum[g] = ww;
     }
 }
 

@@ -33,45 +33,22 @@ __device__ __forceinline__ double gauge_weight3(const GaugeSpec3& G, uint32_t g,
     return u > 0.0 ? u : 0.0;
 }
 
-// Per-block partial sums (sum w e, sum w) of every gauge.
+// Block partial sums (sum w e, sum w) of every gauge (layout of
+// gauge_block_sums).
 __global__ void __launch_bounds__(kBlock) smooth_gauge_partials3(uint32_t n, const double4* __restrict__ pos,
                                                                  GaugeSpec3 G, double* __restrict__ partials) {
-    __shared__ double s_z[kBlock / kWave][kMaxGauges], s_w[kBlock / kWave][kMaxGauges];
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    const uint32_t wave = threadIdx.x / kWave;
     bool in = false;
     double4 p = make_double4(0, 0, 0, 0);
     if (s < n) {
         p = pos[s];
         in = (int)p.w == G.g.ptype;
     }
-    const double inv_r2 = 1.0 / (G.g.radius * G.g.radius);
-    for (uint32_t g = 0; g < G.g.n; ++g) {
-        double dx, dz;
-        const double u = gauge_weight3(G, g, p.x, p.z, inv_r2, dx, dz);
-        const double w = in ? u * u * u : 0.0;
-        double z = w > 0.0 ? w * exp((p.y - G.g.y_ref) / G.g.tau) : 0.0;
-        double ww = w;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            z += __shfl_xor(z, off);
-            ww += __shfl_xor(ww, off);
-        }
-        if (__lane_id() == 0) {
-            s_z[wave][g] = z;
-            s_w[wave][g] = ww;
-        }
-    }
-    __syncthreads();
-    for (uint32_t g = threadIdx.x; g < G.g.n; g += kBlock) {
-        double z = 0, ww = 0;
-        for (uint32_t w = 0; w < kBlock / kWave; ++w) {
-            z += s_z[w][g];
-            ww += s_w[w][g];
-        }
-        partials[(size_t)blockIdx.x * 2 * kMaxGauges + g] = z;
-        partials[(size_t)blockIdx.x * 2 * kMaxGauges + kMaxGauges + g] = ww;
-    }
+    auto d2 = [&](uint32_t g) {
+        const double dx = p.x - G.g.x[g], dz = p.z - G.z[g];
+        return dx * dx + dz * dz;
+    };
+    gauge_block_sums(G.g, in, p.y, d2, 1.0 / (G.g.radius * G.g.radius), partials);
 }
 
 // lam[oid].r += sum_g eta_bar[g] d eta_g / d r (state in sorted order).

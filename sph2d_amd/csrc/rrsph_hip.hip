@@ -782,21 +782,15 @@ static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
 
 static uint64_t committed_steps(const Ctrl& c) { return c.step + (c.pending ? 1 : 0); }
 
-// After a synchronised launch: if a step found more changed slots than the
-// incremental re-sort takes, it halted before touching the state; clear the
-// halt and re-run it (and the steps after it) starting with the full sort.
-static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t nsteps, uint32_t* left) {
-    Ctrl& c = *ctx->host_ctrl;
-    *left = 0;
-    if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
+// A step overflowed the incremental re-sort at step index `at`.  Overflows
+// in close succession mean many particles change cell every step: keep to
+// the full sort for a while instead of paying a host round trip (and the
+// no-op tail of the launched batch) per step, then try again.  A lone
+// overflow, e.g. the first step after an upload, costs one round trip.
+// Distances are in committed steps, not launches: a halted batch still
+// counts its no-op launches.
+static void note_resort_overflow(rrsph_ctx* ctx, uint64_t at) {
     ++ctx->n_fallback;
-    // Overflows in close succession mean many particles change cell every
-    // step: keep to the full sort for a while instead of paying a host round
-    // trip (and the no-op tail of the launched batch) per step, then try
-    // again.  A lone overflow, e.g. the first step after an upload, costs one
-    // round trip.  Distances are in committed steps, not launches: a halted
-    // batch still counts its no-op launches.
-    const uint64_t at = committed_steps(c);  // the step that overflowed
     if (ctx->last_fallback != ~0ull && at - ctx->last_fallback <= kFallbackWindow) {
         const bool again = ctx->full_run && at - ctx->full_run_end <= kFallbackWindow;
         ctx->full_run = again ? std::min(2 * ctx->full_run, kFullSortRunMax) : kFullSortRun;
@@ -804,11 +798,22 @@ static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t
         ctx->full_run_end = at + ctx->full_run;
     }
     ctx->last_fallback = at;
+    ctx->need_full = true;  // the overflowing step itself runs with the full sort
+}
+
+// After a synchronised launch: if a step found more changed slots than the
+// incremental re-sort takes, it halted before touching the state; clear the
+// halt and re-run it (and the steps after it) starting with the full sort.
+static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t nsteps, uint32_t* left) {
+    Ctrl& c = *ctx->host_ctrl;
+    *left = 0;
+    if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
+    const uint64_t at = committed_steps(c);  // the step that overflowed
+    note_resort_overflow(ctx, at);
     *left = nsteps - (uint32_t)(at - first);
     c.halted = 0;
     c.status = 0;
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
-    ctx->need_full = true;
     return RRSPH_OK;
 }
 
@@ -1177,7 +1182,7 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, d
     else
         smooth_gauge_partials<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, G.g,
                                                                partials);
-    smooth_gauge_final<<<1, kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws);
+    smooth_gauge_final<<<std::max(G.g.n, 1u), kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws);
     HIP_TRY(hipGetLastError());
     return RRSPH_OK;
 }
@@ -1241,9 +1246,11 @@ static rrsph_status restore_state_sync(rrsph_ctx* ctx, const void* src) {
 }
 
 // Launch steps [k0, k1) of the tape without host synchronisation: every
-// step's record, the checkpoints (forward) or segment states (replay), and
-// the gauges (forward).  Returns the status read after the last step.
-static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base) {
+// step's record, the checkpoints (forward) or segment states (replay; slot
+// k - kslot of seg_base), and the gauges (forward).  Returns the status read
+// after the last step.
+static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base,
+                                uint32_t kslot) {
     AdjTape& T = ctx->adj;
     for (uint32_t k = k0; k < k1; ++k) {
         rrsph_status st;
@@ -1252,7 +1259,7 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
             if (st != RRSPH_OK) return st;
         }
         if (!forward) {
-            st = save_state(ctx, (char*)seg_base + (size_t)(k - k0) * T.blob);
+            st = save_state(ctx, (char*)seg_base + (size_t)(k - kslot) * T.blob);
             if (st != RRSPH_OK) return st;
             if (k + 1 == k1) break;  // the state before the segment's last step is all the adjoint needs
         }
@@ -1268,21 +1275,31 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
     return sync_status(ctx);
 }
 
-// adjoint_run with the incremental re-sort; if a step needed the full-sort
-// fallback (a host round trip), redo the range with the full sort only.
-static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base,
-                                        const void* start) {
-    rrsph_status st = adjoint_run(ctx, k0, k1, forward, seg_base);
-    if (st != RRSPH_OK) return st;
-    if (!(ctx->host_ctrl->halted && ctx->host_ctrl->status == kStatusResort)) return RRSPH_OK;
-    ++ctx->n_fallback;
-    st = restore_state_sync(ctx, start);
-    if (st != RRSPH_OK) return st;
-    const bool inc = ctx->incremental;
-    ctx->incremental = false;
-    st = adjoint_run(ctx, k0, k1, forward, seg_base);
-    ctx->incremental = inc;
-    return st;
+// adjoint_run with the incremental re-sort.  A step that changed more cells
+// than the re-sort takes halts before touching the state, and the launches
+// after it do nothing; clear the halt and resume the range from that step,
+// which runs with the full sort (its record, checkpoint, segment state and
+// gauges are written again).
+static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base) {
+    HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint64_t start = committed_steps(*ctx->host_ctrl);
+    uint32_t from = k0;
+    while (true) {
+        rrsph_status st = adjoint_run(ctx, from, k1, forward, seg_base, k0);
+        if (st != RRSPH_OK) return st;
+        Ctrl& c = *ctx->host_ctrl;
+        if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
+        const uint32_t at = k0 + (uint32_t)(committed_steps(c) - start);  // the step that overflowed
+        // the resumed step runs with the full sort, so every resume advances
+        if (at < from || at >= k1 || (at == from && from != k0))
+            return fail(ctx, RRSPH_ERR_HIP, "RRSPH_HIP: re-sort overflow outside the replayed range");
+        note_resort_overflow(ctx, committed_steps(c));
+        c.halted = 0;
+        c.status = 0;
+        HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+        from = at;
+    }
 }
 
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
@@ -1371,7 +1388,7 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
         free_tape(ctx->adj);
         return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
     }
-    st = adjoint_run_checked(ctx, 0, nsteps, true, nullptr, T.ck[0]);
+    st = adjoint_run_checked(ctx, 0, nsteps, true, nullptr);
     if (st != RRSPH_OK) return st;
     if (ctx->host_ctrl->halted)
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
@@ -1434,7 +1451,7 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         // replay the segment from its checkpoint, keeping the state before each step
         st = restore_state(ctx, T.ck[seg]);
         if (st != RRSPH_OK) return st;
-        st = adjoint_run_checked(ctx, n0, n1, false, T.seg, T.ck[seg]);
+        st = adjoint_run_checked(ctx, n0, n1, false, T.seg);
         if (st != RRSPH_OK) return st;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
             st = restore_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
