@@ -87,6 +87,7 @@ struct AdjTape {
     size_t blob = 0;                     // bytes of one saved state
     std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
     void* seg = nullptr;                 // states of one replayed segment (every blobs)
+    bool seg_last = false;               // seg holds the last segment's states (kept by the forward)
     AdjStep* rec = nullptr;              // per step (device)
     std::vector<AdjStep> hrec;           // per step (host copy)
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
@@ -1258,6 +1259,13 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
             st = save_state(ctx, T.ck[k / T.every]);
             if (st != RRSPH_OK) return st;
         }
+        // the forward keeps the states of the last segment: the reverse pass
+        // starts there and needs no replay of it
+        const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
+        if (forward && k >= last0) {
+            st = save_state(ctx, (char*)T.seg + (size_t)(k - last0) * T.blob);
+            if (st != RRSPH_OK) return st;
+        }
         if (!forward) {
             st = save_state(ctx, (char*)seg_base + (size_t)(k - kslot) * T.blob);
             if (st != RRSPH_OK) return st;
@@ -1394,6 +1402,7 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
     st = save_state(ctx, T.ck[nck]);
     if (st != RRSPH_OK) return st;
+    T.seg_last = true;
     std::vector<double> h((size_t)nsteps * kMaxGauges);
     HIP_TRY(hipMemcpyAsync(h.data(), T.eta, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     T.hrec.resize(nsteps);
@@ -1448,11 +1457,15 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
     int cur = 0;
     for (int seg = (int)nck - 1; seg >= 0; --seg) {
         const uint32_t n0 = (uint32_t)seg * T.every, n1 = std::min(n0 + T.every, N);
-        // replay the segment from its checkpoint, keeping the state before each step
-        st = restore_state(ctx, T.ck[seg]);
-        if (st != RRSPH_OK) return st;
-        st = adjoint_run_checked(ctx, n0, n1, false, T.seg);
-        if (st != RRSPH_OK) return st;
+        // replay the segment from its checkpoint, keeping the state before each
+        // step (the last segment's states are there already after a forward)
+        if (!(T.seg_last && seg == (int)nck - 1)) {
+            st = restore_state(ctx, T.ck[seg]);
+            if (st != RRSPH_OK) return st;
+            st = adjoint_run_checked(ctx, n0, n1, false, T.seg);
+            if (st != RRSPH_OK) return st;
+        }
+        T.seg_last = false;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
             st = restore_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
             if (st != RRSPH_OK) return st;
