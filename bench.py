@@ -149,11 +149,13 @@ def adjoint_leg(steps, every):
 
 def adjoint3d_leg(steps, every, delta, nfd):
     """The differentiable 3D piston basin (6 m x 2 m, depth 0.5 m, 8 paddles
-    x 4 harmonics = 64 coefficients), FP64: plain forward steps, the taped
-    forward and the reverse pass on the wall clock, and the gradient of the
-    gauge loss against central differences on `nfd` sampled coefficients.
-    The 4 x 4 gauge grid sits within the distance sound travels in the run,
-    so every gauge sees the paddles."""
+    x 4 harmonics = 64 coefficients): for the FP32 step (FP32 pairs, FP64
+    positions; adjoint arithmetic FP64 on its trajectory) and the FP64 step,
+    plain forward steps, the taped forward and the reverse pass on the wall
+    clock; then both gradients of the gauge loss against FP64 central
+    differences on `nfd` sampled coefficients.  The 4 x 4 gauge grid sits
+    within the distance sound travels in the run, so every gauge sees the
+    paddles."""
     import numpy as np
     import torch
 
@@ -168,41 +170,44 @@ def adjoint3d_leg(steps, every, delta, nfd):
     zs = [0.25, 0.75, 1.25, 1.75]
     gx, gz = [x for x in xs for _ in zs], [z for _ in xs for z in zs]
     spec = H.gauge_spec(gx, radius=3 * delta, tau=delta / 4, y_ref=prm.depth, zs=gz)
-    g = H.HipStep(prm, precision=H.FP64)
     a, b = np.array(prm.piston_a), np.array(prm.piston_b)
-    g.upload(lay.r, lay.v, lay.rho, lay.itype)
-    g.step(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    g.step(steps)
-    torch.cuda.synchronize()
-    fwd = time.perf_counter() - t0
-    run = DifferentiableRun(g, dict(lay.state), steps, every, spec, prm.piston_omega)
-    t0 = time.perf_counter()
-    eta = run.forward(a, b)
-    t1 = time.perf_counter()
     loss = lambda e: float(np.sum((e - prm.depth) ** 2))
-    grads = run.backward(2.0 * (eta - prm.depth), len(a))
-    t2 = time.perf_counter()
-    errs = []
-    rng = np.random.default_rng(1)
-    for j in rng.choice(2 * len(a), size=nfd, replace=False):
-        which, k = ("a", int(j)) if j < len(a) else ("b", int(j) - len(a))
-        fd = central_difference(run, a, b, loss, which, k, 1e-6)
-        errs.append({"coef": f"{which}[{k}]", "adjoint": grads[which][k], "fd": fd,
-                     "rel_err": abs(grads[which][k] - fd) / max(abs(fd), 1e-300)})
-    g.close()
-    return {
-        "workload": "3D piston basin 6 m x 2 m, depth 0.5 m, delta %g m, %d particles (%d fluid), 8 paddles x 4 "
-                    "harmonics, 4 x 4 gauges, %d steps, checkpoint every %d, FP64" % (delta, prm.ntotal, prm.nfluid,
-                                                                                     steps, every),
-        "forward_s": round(fwd, 4),
-        "taped_forward_s": round(t1 - t0, 4),
-        "backward_s": round(t2 - t1, 4),
-        "adjoint_over_forward": round((t2 - t1) / fwd, 3),
-        "fd_check": errs,
-        "fd_max_rel_err": max(e["rel_err"] for e in errs) if errs else None,
-    }
+    out = {"workload": "3D piston basin 6 m x 2 m, depth 0.5 m, delta %g m, %d particles (%d fluid), 8 paddles x 4 "
+                       "harmonics, 4 x 4 gauges, %d steps, checkpoint every %d" % (delta, prm.ntotal, prm.nfluid,
+                                                                                steps, every)}
+    grads = {}
+    for name, prec in (("fp32", H.FP32), ("fp64", H.FP64)):
+        g = H.HipStep(prm, precision=prec)
+        g.upload(lay.r, lay.v, lay.rho, lay.itype)
+        g.step(1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.step(steps)
+        torch.cuda.synchronize()
+        fwd = time.perf_counter() - t0
+        run = DifferentiableRun(g, dict(lay.state), steps, every, spec, prm.piston_omega)
+        t0 = time.perf_counter()
+        eta = run.forward(a, b)
+        t1 = time.perf_counter()
+        grads[name] = run.backward(2.0 * (eta - prm.depth), len(a))
+        t2 = time.perf_counter()
+        out[name] = {"forward_s": round(fwd, 4), "taped_forward_s": round(t1 - t0, 4),
+                     "backward_s": round(t2 - t1, 4), "adjoint_over_forward": round((t2 - t1) / fwd, 3)}
+        if name == "fp64":
+            errs = []
+            rng = np.random.default_rng(1)
+            for j in rng.choice(2 * len(a), size=nfd, replace=False):
+                which, k = ("a", int(j)) if j < len(a) else ("b", int(j) - len(a))
+                fd = central_difference(run, a, b, loss, which, k, 1e-6)
+                errs.append({"coef": f"{which}[{k}]", "fd_fp64": fd, "adjoint_fp64": grads["fp64"][which][k],
+                             "adjoint_fp32": grads["fp32"][which][k],
+                             "rel_err_fp64": abs(grads["fp64"][which][k] - fd) / max(abs(fd), 1e-300),
+                             "rel_err_fp32": abs(grads["fp32"][which][k] - fd) / max(abs(fd), 1e-300)})
+            out["fd_check"] = errs
+            out["fd_max_rel_err_fp64"] = max(e["rel_err_fp64"] for e in errs) if errs else None
+            out["fd_max_rel_err_fp32"] = max(e["rel_err_fp32"] for e in errs) if errs else None
+        g.close()
+    return out
 
 
 def leg_3d(steps, delta):

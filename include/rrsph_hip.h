@@ -199,7 +199,9 @@ rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t n
 
 /* ---- differentiation ------------------------------------------------- */
 /* No reference counterpart: the reference has no derivative of its step.
- * The adjoint differentiates the FP64 step, 2D or 3D (continuity density,
+ * The adjoint differentiates the step, 2D or 3D, in FP64 arithmetic on the
+ * trajectory of the context's precision (an FP32 context tapes and replays
+ * the FP32 step) (continuity density,
  * cubic kernels, pressure, artificial viscosity, Lennard-Jones walls, XSPH,
  * boundary motion with the Fourier piston, nwm = 6, one piston or paddles
  * in 3D) with dt independent of the state (dt_correction_method 0, or 2

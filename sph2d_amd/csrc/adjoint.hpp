@@ -37,6 +37,12 @@ namespace rrsph {
 constexpr uint32_t kMaxGauges = 64;
 constexpr uint32_t kAdjChunk = 32;
 
+// The adjoint runs in FP64 on the state of either precision (an FP32 state
+// is the FP32 step's trajectory; its Jacobian is taken in FP64).
+template <class R> __device__ __forceinline__ double4 to_d4(const real4<R>& v) {
+    return make_double4((double)v.x, (double)v.y, (double)v.z, (double)v.w);
+}
+
 struct GaugeSpec {
     uint32_t n;          // gauges
     int32_t ptype;       // particle type the gauge looks at
@@ -90,8 +96,9 @@ __device__ __forceinline__ void gauge_block_sums(const GaugeSpec& G, bool in, do
     }
 }
 
+template <class R>
 __global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, const double2* __restrict__ pos,
-                                                                const double4* __restrict__ va, GaugeSpec G,
+                                                                const real4<R>* __restrict__ va, GaugeSpec G,
                                                                 double* __restrict__ partials) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     bool in = false;
@@ -147,8 +154,9 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_final(uint32_t nblocks, G
 }
 
 // lam[oid].r += sum_g eta_bar[g] d eta_g / d r (state in sorted order).
+template <class R>
 __global__ void __launch_bounds__(kBlock) smooth_gauge_grad(uint32_t n, const double2* __restrict__ pos,
-                                                            const double4* __restrict__ va,
+                                                            const real4<R>* __restrict__ va,
                                                             const uint32_t* __restrict__ oid, GaugeSpec G,
                                                             const double* __restrict__ eta_bar,
                                                             const double* __restrict__ zsum,
@@ -177,8 +185,9 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_grad(uint32_t n, const do
 }
 
 // Cell keys of a restored FP64 state (dead particles get the dead key).
+template <class R>
 __global__ void __launch_bounds__(kBlock) state_keys_kernel(StepParams P, const double2* __restrict__ pos,
-                                                            const double4* __restrict__ va,
+                                                            const real4<R>* __restrict__ va,
                                                             uint32_t* __restrict__ key_next) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= P.n) return;
@@ -222,7 +231,8 @@ __global__ void adj_util_ctrl(Ctrl* __restrict__ uc, const AdjStep* __restrict__
 // whole_step reverse.  out_adj[s] = (a_x, a_y, av_x, av_y) adjoints,
 // drho_adj[s]; direct[s] = (r_x, r_y, v_x, v_y, rho) adjoints that do not
 // pass through the pair sums; piston_sum += wave-maker velocity adjoint.
-__global__ void __launch_bounds__(kBlock) adj_prep(StepParams P, AdjStep T, StateB<double> B,
+template <class R>
+__global__ void __launch_bounds__(kBlock) adj_prep(StepParams P, AdjStep T, StateB<R> B,
                                                    const double* __restrict__ lam_in,
                                                    double4* __restrict__ out_adj, double* __restrict__ drho_adj,
                                                    double* __restrict__ direct, double* __restrict__ piston_sum) {
@@ -355,7 +365,8 @@ __device__ __forceinline__ void adj_orient(const AdjConsts& K, const AdjPair& Q,
     }
 }
 
-__global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, StateB<double> B,
+template <class R>
+__global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, StateB<R> B,
                                                    const uint32_t* __restrict__ skey,
                                                    const uint32_t* __restrict__ LB,
                                                    const double4* __restrict__ out_adj,
@@ -383,8 +394,8 @@ __global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, Stat
     K.xsph_on = P.xsph_on;
     K.lj_on = P.lj_on;
     const double2 pk = B.pos[s];
-    const double4 vak = B.va[s];
-    const double4 prk = B.pred[s];
+    const double4 vak = to_d4<R>(B.va[s]);
+    const double4 prk = to_d4<R>(B.pred[s]);
     const double pres_k = B.pres[s];
     const double4 oak = out_adj[s];
     const double dak = drho_adj[s];
@@ -398,15 +409,15 @@ __global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, Stat
             for (uint32_t u = 0; u < nh; ++u) {
                 const uint32_t i = my[u * kBlock];
                 const double2 pi = B.pos[i];
-                const double4 vai = B.va[i];
-                const double4 pri = B.pred[i];
+                const double4 vai = to_d4<R>(B.va[i]);
+                const double4 pri = to_d4<R>(B.pred[i]);
                 const double4 oai = out_adj[i];
                 const double dai = drho_adj[i];
                 const int ti = (int)vai.w;
                 const double ex = pi.x - pk.x, ey = pi.y - pk.y;  // r_i - r_k
                 AdjPair Q;
                 Q.d2 = ex * ex + ey * ey;
-                Q.rinv = Q.d2 > 0.0 ? 1.0 / sqrt(Q.d2) : 0.0;
+                Q.rinv = Q.d2 > 0.0 ? fast_rsqrt<double>(Q.d2) : 0.0;
                 Q.d = Q.d2 * Q.rinv;
                 const double q = Q.d * K.inv_h;
                 const double t = 2.0 - q;
@@ -419,10 +430,10 @@ __global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, Stat
                     Q.Fp = K.c2 * t * (2.0 * Q.rinv * K.inv_h + t * Q.rinv * Q.rinv);
                     Q.W = K.w2 * t * t * t;
                 }
-                Q.IR = 1.0 / (pri.z + prk.z);
+                Q.IR = fast_rcp<double>(pri.z + prk.z);
                 const double dux = pri.x - prk.x, duy = pri.y - prk.y;  // u_i - u_k
                 Q.vr = dux * ex + duy * ey;
-                Q.den = 1.0 / (Q.d2 + K.eta2);
+                Q.den = fast_rcp<double>(Q.d2 + K.eta2);
                 Q.mu = K.h * Q.vr * Q.den;
                 Q.app = K.av_on && Q.vr < 0.0;
                 Q.A = Q.app ? 2.0 * Q.IR * (K.beta * Q.mu * Q.mu - K.alpha_c * Q.mu) : 0.0;

@@ -96,7 +96,8 @@ __global__ void __launch_bounds__(kBlock) state_keys3_kernel(StepParams P, const
 // whole_step reverse (WholeStep.cpp:19-61 with the boundary motion of
 // cl/TimeIntegration.cl:190-245).  piston_sum[g] += adjoint of paddle g's
 // velocity (one entry per paddle of this step).
-__global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, StateB3<double> B,
+template <class R>
+__global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, StateB3<R> B,
                                                     const double* __restrict__ lam_in, double4* __restrict__ out_a,
                                                     double4* __restrict__ out_x, double* __restrict__ direct,
                                                     double* __restrict__ piston_sum) {
@@ -237,7 +238,8 @@ __device__ __forceinline__ void adj_orient3(const AdjConsts& K, const AdjPair& Q
 // Gather over neighbours, both orientations of every pair, then the
 // predict and EOS reverse; writes lam_out.  Every lane of a block runs the
 // wave-uniform walk (lanes past n walk empty ranges).
-__global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, StateB3<double> B,
+template <class R>
+__global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, StateB3<R> B,
                                                     const uint32_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ LB,
                                                     const double4* __restrict__ out_a,
@@ -266,9 +268,9 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     K.xsph_on = P.xsph_on;
     K.lj_on = P.lj_on;
     const double4 pk = B.pos[sc];
-    const double4 vak = B.va[sc];    // (v, rho)
-    const double4 prk = B.pred[sc];  // (u, R)
-    const double Pk = B.pw[sc];      // p / R^2
+    const double4 vak = to_d4<R>(B.va[sc]);    // (v, rho)
+    const double4 prk = to_d4<R>(B.pred[sc]);  // (u, R)
+    const double Pk = B.pw[sc];                // p / R^2
     const double pres_k = B.pres[sc];
     const double4 oak = out_a[sc];
     const double4 oxk = out_x[sc];
@@ -277,21 +279,36 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     double Rb = 0, Pb = 0;
     bool overflow = false;
     uint32_t* my = hits + threadIdx.x;
+    // the loads of hit u + 1 are issued before the arithmetic of hit u (the
+    // kernel runs at 2 waves per SIMD either way, so the registers are free)
     auto flush = [&](uint32_t nh) {
+        if (nh == 0) return;
+        const uint32_t last = nh - 1;
+        uint32_t in = my[0];
+        double4 pn = B.pos[in];
+        real4<R> van = B.va[in], prn = B.pred[in];
+        R pwn = B.pw[in];
+        double4 oan = out_a[in], oxn = out_x[in];
 #pragma unroll 1
         for (uint32_t u = 0; u < nh; ++u) {
-            const uint32_t i = my[u * kBlock];
-            const double4 pi = B.pos[i];
-            const double4 vai = B.va[i];
-            const double4 pri = B.pred[i];
-            const double Pi = B.pw[i];
-            const double4 oai = out_a[i];
-            const double4 oxi = out_x[i];
+            const double4 pi = pn;
+            const double4 vai = to_d4<R>(van);
+            const double4 pri = to_d4<R>(prn);
+            const double Pi = pwn;
+            const double4 oai = oan;
+            const double4 oxi = oxn;
+            in = my[min(u + 1, last) * kBlock];
+            pn = B.pos[in];
+            van = B.va[in];
+            prn = B.pred[in];
+            pwn = B.pw[in];
+            oan = out_a[in];
+            oxn = out_x[in];
             const int ti = (int)pi.w;
             const V3 e{pi.x - pk.x, pi.y - pk.y, pi.z - pk.z};  // r_i - r_k
             AdjPair Q;
             Q.d2 = dot3(e, e);
-            Q.rinv = Q.d2 > 0.0 ? 1.0 / sqrt(Q.d2) : 0.0;
+            Q.rinv = Q.d2 > 0.0 ? fast_rsqrt<double>(Q.d2) : 0.0;
             Q.d = Q.d2 * Q.rinv;
             const double q = Q.d * K.inv_h;
             const double t = 2.0 - q;
@@ -304,10 +321,10 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
                 Q.Fp = K.c2 * t * (2.0 * Q.rinv * K.inv_h + t * Q.rinv * Q.rinv);
                 Q.W = K.w2 * t * t * t;
             }
-            Q.IR = 1.0 / (pri.w + prk.w);
+            Q.IR = fast_rcp<double>(pri.w + prk.w);
             const V3 du{pri.x - prk.x, pri.y - prk.y, pri.z - prk.z};  // u_i - u_k
             Q.vr = dot3(du, e);
-            Q.den = 1.0 / (Q.d2 + K.eta2);
+            Q.den = fast_rcp<double>(Q.d2 + K.eta2);
             Q.mu = K.h * Q.vr * Q.den;
             Q.app = K.av_on && Q.vr < 0.0;
             Q.A = Q.app ? 2.0 * Q.IR * (K.beta * Q.mu * Q.mu - K.alpha_c * Q.mu) : 0.0;

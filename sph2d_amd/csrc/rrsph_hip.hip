@@ -616,6 +616,29 @@ void free_all(rrsph_ctx* c) {
 
 }  // namespace
 
+// The reverse of one step on its re-sorted state (state precision R, adjoint
+// arithmetic FP64): whole_step reverse, then the pair gather.
+template <class R>
+static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double* lam_in, double* lam_out,
+                                 double* psum) {
+    AdjTape& T = ctx->adj;
+    const StepParams& P = ctx->P;
+    const size_t n = P.n;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    enqueue_util_sort<R>(ctx);
+    if (P.dim == 3) {
+        StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
+        adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.out_adj + n, T.direct, psum);
+        adj_pair3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.out_adj + n,
+                                                     T.direct, lam_out, T.flag);
+    } else {
+        StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
+        adj_prep<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.drho_adj, T.direct, psum);
+        adj_pair<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
+                                                    T.direct, lam_out, T.flag);
+    }
+}
+
 extern "C" {
 
 rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
@@ -1180,9 +1203,12 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, d
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (ctx->P.dim == 3)
         smooth_gauge_partials3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, G, partials);
+    else if (ctx->fp32)
+        smooth_gauge_partials<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const float4*)ctx->vaA,
+                                                                      G.g, partials);
     else
-        smooth_gauge_partials<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, G.g,
-                                                               partials);
+        smooth_gauge_partials<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA,
+                                                                       G.g, partials);
     smooth_gauge_final<<<std::max(G.g.n, 1u), kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws);
     HIP_TRY(hipGetLastError());
     return RRSPH_OK;
@@ -1190,7 +1216,6 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, d
 
 static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
     const StepParams& P = ctx->P;
-    if (ctx->fp32) return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: the adjoint needs an FP64 context");
     if (!ctx->fast_pair || P.density_treatment != 1)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: the adjoint needs continuity density, cubic kernels, approximation 2 and no "
@@ -1204,34 +1229,45 @@ static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
 
 // one saved state: posA | vaA | adA | oidA | ctrl
 // bytes per particle of the FP64 position array (2D double2, 3D double4)
+// and of one real4 state array (va, ad: FP32 or FP64)
 static size_t pos_bytes(const rrsph_ctx* ctx) { return ctx->P.dim == 3 ? 32 : 16; }
+static size_t real4_bytes(const rrsph_ctx* ctx) { return ctx->fp32 ? 16 : 32; }
+// bytes of one saved state: pos | va | ad | oid | control block
+static size_t state_bytes(const rrsph_ctx* ctx) {
+    return (pos_bytes(ctx) + 2 * real4_bytes(ctx) + 4) * ctx->P.n + sizeof(Ctrl);
+}
 
 static rrsph_status save_state(rrsph_ctx* ctx, void* dst) {
-    const size_t n = ctx->P.n, pb = pos_bytes(ctx);
+    const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
     char* d = (char*)dst;
     HIP_TRY(hipMemcpyAsync(d, ctx->posA, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + pb * n, ctx->vaA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + 32) * n, ctx->adA, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + 64) * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + 68) * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + pb * n, ctx->vaA, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + vb) * n, ctx->adA, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + 2 * vb) * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(d + (pb + 2 * vb + 4) * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice,
+                           ctx->stream));
     return RRSPH_OK;
 }
 
 static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
-    const size_t n = ctx->P.n, pb = pos_bytes(ctx);
+    const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
     const char* d = (const char*)src;
     HIP_TRY(hipMemcpyAsync(ctx->posA, d, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + pb * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->adA, d + (pb + 32) * n, 32 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + (pb + 64) * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + (pb + 68) * n, sizeof(Ctrl), hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + pb * n, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->adA, d + (pb + vb) * n, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + (pb + 2 * vb) * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + (pb + 2 * vb + 4) * n, sizeof(Ctrl), hipMemcpyDeviceToDevice,
+                           ctx->stream));
     // the next step's cell keys from the restored positions, and a full sort
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (ctx->P.dim == 3)
         state_keys3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->key_next);
+    else if (ctx->fp32)
+        state_keys_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const float4*)ctx->vaA,
+                                                                 ctx->key_next);
     else
-        state_keys_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
-                                                          ctx->key_next);
+        state_keys_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
+                                                                  ctx->key_next);
     HIP_TRY(hipGetLastError());
     ctx->need_full = true;
     return RRSPH_OK;
@@ -1312,8 +1348,6 @@ static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1
 
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
     if (!ctx || !ctx->uploaded || !eta) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
-    if (ctx->fp32 && ctx->P.dim == 2)
-        return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED, "RRSPH_HIP: 2D smooth gauges need an FP64 context");
     GaugeSpec3 G;
     rrsph_status st = to_gauge_spec(ctx, spec, G);
     if (st != RRSPH_OK) return st;
@@ -1368,7 +1402,7 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
     T.nsteps = nsteps;
     T.every = std::min(checkpoint_every, nsteps);
     T.spec = G;
-    T.blob = (((pos_bytes(ctx) + 68) * n + sizeof(Ctrl)) + 255) / 256 * 256;
+    T.blob = (state_bytes(ctx) + 255) / 256 * 256;
     const uint32_t nck = (nsteps + T.every - 1) / T.every;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     hipError_t e = hipSuccess;
@@ -1422,9 +1456,14 @@ static void enqueue_gauge_grad(rrsph_ctx* ctx, uint32_t k, double* lam) {
     if (ctx->P.dim == 3)
         smooth_gauge_grad3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, ctx->oidA, T.spec,
                                                            T.eta_bar + o, T.zs + o, T.ws + o, lam);
+    else if (ctx->fp32)
+        smooth_gauge_grad<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const float4*)ctx->vaA,
+                                                                 ctx->oidA, T.spec.g, T.eta_bar + o, T.zs + o,
+                                                                 T.ws + o, lam);
     else
-        smooth_gauge_grad<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA, ctx->oidA,
-                                                          T.spec.g, T.eta_bar + o, T.zs + o, T.ws + o, lam);
+        smooth_gauge_grad<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA,
+                                                                  ctx->oidA, T.spec.g, T.eta_bar + o, T.zs + o,
+                                                                  T.ws + o, lam);
 }
 
 rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
@@ -1471,22 +1510,12 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
             if (st != RRSPH_OK) return st;
             // sort and reorder state k exactly as its step did (predict with its dt)
             adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
-            enqueue_util_sort<double>(ctx);
-            const AdjStep& S = T.hrec[k];
-            double* psum = T.piston_sum + (size_t)k * kMaxSegments;
-            if (d3) {
-                StateB3<double> B = stateB3<double>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
-                adj_prep3<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.out_adj + n, T.direct,
-                                                          psum);
-                adj_pair3<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj,
-                                                          T.out_adj + n, T.direct, T.lam[1 - cur], T.flag);
-            } else {
-                StateB<double> B = stateB<double>(ctx, ctx->pred_scratch, ctx->pres_scratch);
-                adj_prep<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, T.lam[cur], T.out_adj, T.drho_adj, T.direct,
-                                                         psum);
-                adj_pair<<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
-                                                         T.direct, T.lam[1 - cur], T.flag);
-            }
+            if (ctx->fp32)
+                enqueue_reverse_step<float>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
+                                            T.piston_sum + (size_t)k * kMaxSegments);
+            else
+                enqueue_reverse_step<double>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
+                                             T.piston_sum + (size_t)k * kMaxSegments);
             if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur]);  // loss term of state k
             HIP_TRY(hipGetLastError());
             cur = 1 - cur;

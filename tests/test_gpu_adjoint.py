@@ -215,9 +215,36 @@ def test_piston_gradient_matches_oracle_finite_differences():
     run.sim.close()
 
 
-def test_adjoint_rejects_fp32_and_summation():
+def test_fp32_trajectory_adjoint_matches_the_fp64_gradient():
+    """Mixed precision: the adjoint on the FP32 step's trajectory (FP32 pair
+    arithmetic, FP64 positions; adjoint arithmetic FP64) against the FP64
+    adjoint, which central differences pin to 1e-6: within 1e-3 relative on
+    every coefficient above a thousandth of the largest."""
+    lay, run64 = _case(nsteps=40, every=10)
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    eta64 = run64.forward(a, b)
+    c = _weights(eta64.shape, seed=5)
+    g64 = run64.backward(c, len(a))
+    run64.sim.close()
+    sim32 = H.HipStep(prm, precision=H.FP32)
+    run32 = DifferentiableRun(sim32, dict(lay.state), run64.nsteps, run64.every, run64.spec, prm.piston_omega)
+    eta32 = run32.forward(a, b)
+    assert np.max(np.abs(eta32 - eta64)) < 1e-5
+    g32 = run32.backward(c, len(a))
+    sim32.close()
+    for which in ("a", "b"):
+        big = np.abs(g64[which]) > 1e-3 * np.max(np.abs(g64[which]))
+        err = np.abs(g32[which] - g64[which])[big] / np.abs(g64[which])[big]
+        print(which, "fp32 vs fp64 max rel", float(err.max()), flush=True)
+        assert err.max() < 1e-3, (which, err.max())
+
+
+def test_adjoint_rejects_summation():
     lay, _ = _case()
-    sim = H.HipStep(lay.params, precision=H.FP32)
+    prm = copy.copy(lay.params)
+    prm.density_treatment = 0
+    sim = H.HipStep(prm, precision=H.FP64)
     sim.upload(lay.r, lay.v, lay.rho, lay.itype)
     spec = H.gauge_spec([0.1], radius=0.06, tau=0.005, y_ref=0.3)
     with pytest.raises(H.RrsphError) as e:

@@ -191,3 +191,27 @@ def test_3d_paddle_gradient_matches_oracle_finite_differences():
         assert abs(fd) > 1e-4, (which, k, fd)
         assert abs(g[which][k] - fd) <= 1e-3 * abs(fd), (which, k, g[which][k], fd)
     run.sim.close()
+
+
+def test_3d_fp32_trajectory_adjoint_matches_the_fp64_gradient():
+    """Mixed precision in 3D (the basin with paddles): the adjoint on the FP32
+    trajectory against the FP64 adjoint, within 1e-3 relative on every
+    coefficient above a thousandth of the largest."""
+    lay, run64 = _case(nsteps=20, every=8)
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    eta64 = run64.forward(a, b)
+    c = np.random.default_rng(17).normal(0.0, 1.0, eta64.shape)
+    g64 = run64.backward(c, len(a))
+    run64.sim.close()
+    sim32 = H.HipStep(prm, precision=H.FP32)
+    run32 = DifferentiableRun(sim32, dict(lay.state), run64.nsteps, run64.every, run64.spec, prm.piston_omega)
+    eta32 = run32.forward(a, b)
+    assert np.max(np.abs(eta32 - eta64)) < 1e-5
+    g32 = run32.backward(c, len(a))
+    sim32.close()
+    for which in ("a", "b"):
+        big = np.abs(g64[which]) > 1e-3 * np.max(np.abs(g64[which]))
+        err = np.abs(g32[which] - g64[which])[big] / np.abs(g64[which])[big]
+        print(which, "fp32 vs fp64 max rel", float(err.max()), flush=True)
+        assert err.max() < 1e-3, (which, err.max())
