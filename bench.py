@@ -30,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+VALU_SIMDS = 1024      # 256 CUs x 4 SIMDs
+CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 # Algorithmic bytes per particle for the pair pass (continuity, XSPH, AV,
 # dynamic dt) and for the whole step: the canonical pipeline's compulsory
 # traffic, one read of every input and one write of every output per kernel.
@@ -325,7 +327,16 @@ def main():
     traffic_file = os.path.join(ROOT, "profiles", "pair_kernel_c1_1m_fp32_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
-            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        roofline["traffic"] = prof.get("hbm_bytes_per_launch")
+        # the pair pass is bound by VALU issue: the profiled VALU instructions
+        # of one launch at one instruction per 4 cycles on each of the 1,024
+        # SIMDs (wave64 on 16 lanes), over this run's average launch time
+        valu = prof.get("valu_insts_per_launch")
+        if valu:
+            valu_ms = valu * 4 / (VALU_SIMDS * CLOCK_HZ) * 1e3
+            roofline["valu_issue_bound_ms"] = round(valu_ms, 5)
+            roofline["valu_issue_frac"] = round(valu_ms / pair_ms, 4)
 
     r3d = leg_3d(args.steps_3d, args.delta_3d) if args.steps_3d > 0 else None
 
