@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--delta", type=float, default=None, help="particle spacing (default: the ~1M flume)")
-    ap.add_argument("--cpu-steps", type=int, default=60, help="oracle steps in the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=180, help="oracle steps in the CPU baseline sample (~10 s)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=10)
@@ -252,6 +252,28 @@ def leg_3d(steps, delta):
     return out
 
 
+def copy_bandwidth():
+    """Device-to-device copy of 2 GiB (read + write counted), best of 5, on
+    the box itself: the attainable HBM bandwidth beside the published peak."""
+    import torch
+
+    n = 1 << 31
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = 0.0
+    for _ in range(5):
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record()
+        dst.copy_(src)
+        t1.record()
+        t1.synchronize()
+        best = max(best, 2 * n / (t0.elapsed_time(t1) * 1e-3) / 1e9)
+    del src, dst
+    torch.cuda.empty_cache()
+    return best
+
+
 def precision_error(s32, s64, delta):
     """Error of the FP32 run against the FP64 run after the same steps from
     the same initial state: max relative density error, max position
@@ -337,6 +359,9 @@ def main():
             valu_ms = valu * 4 / (VALU_SIMDS * CLOCK_HZ) * 1e3
             roofline["valu_issue_bound_ms"] = round(valu_ms, 5)
             roofline["valu_issue_frac"] = round(valu_ms / pair_ms, 4)
+
+    roofline["copy_measured_gbs"] = round(copy_bandwidth(), 1)
+    roofline["frac_of_measured_copy"] = round(achieved / roofline["copy_measured_gbs"], 4)
 
     r3d = leg_3d(args.steps_3d, args.delta_3d) if args.steps_3d > 0 else None
 

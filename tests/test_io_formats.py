@@ -1,0 +1,66 @@
+"""The particle dump CSV (Input.cpp:203-236 reading, Output.cpp:81-167
+writing) and the experiment parameter rules, on the CPU."""
+import numpy as np
+import pytest
+
+from helpers import load_dambreak_case
+from sph2d_amd import io as IO
+from sph2d_amd import params as P
+
+
+def test_dump_header_and_six_decimal_values(tmp_path):
+    """make_csv_header order (x, y[, z], itype, vx, vy[, vz], rho, p) and
+    std::to_string's six decimals, itype as an integer."""
+    r = np.array([[0.1, 2.0], [1.0 / 3.0, -0.5]])
+    v = np.array([[1e-7, -2.5], [0.0, 3.25]])
+    path = tmp_path / "0.csv"
+    IO.write_dump(str(path), r, np.array([2, -2]), v=v, rho=np.array([1000.0, 999.9999996]),
+                  p=np.array([0.0, -12.5]))
+    lines = path.read_text().splitlines()
+    assert lines[0] == "x,y,itype,vx,vy,rho,p"
+    assert lines[1] == "0.100000,2.000000,2,0.000000,-2.500000,1000.000000,0.000000"
+    assert lines[2] == "0.333333,-0.500000,-2,0.000000,3.250000,1000.000000,-12.500000"
+    r3 = np.array([[0.1, 0.2, 0.3]])
+    IO.write_dump(str(tmp_path / "3d.csv"), r3, np.array([2]), v=np.zeros((1, 3)), rho=np.ones(1), p=np.zeros(1))
+    assert (tmp_path / "3d.csv").read_text().splitlines()[0] == "x,y,z,itype,vx,vy,vz,rho,p"
+
+
+def test_golden_dump_round_trips_through_the_six_decimal_format(tmp_path):
+    """The reference's dam-break dump (six-decimal values as parsed) written
+    and read back gives the same doubles."""
+    prm, st, _ = load_dambreak_case()
+    path = str(tmp_path / "0.00.csv")
+    IO.write_dump(path, st["r"], st["itype"], v=st["v"], rho=st["rho"], p=np.zeros(len(st["rho"])))
+    back = IO.read_dump(path, dim=2, ntotal=prm.ntotal)
+    for k in ("r", "v", "rho", "itype"):
+        assert np.array_equal(back[k], st[k]), k
+
+
+def test_lossless_dump_round_trips_bit_exactly(tmp_path):
+    rng = np.random.default_rng(3)
+    r = rng.normal(size=(50, 3))
+    v = rng.normal(size=(50, 3)) * 1e-9
+    rho = 1000 + rng.normal(size=50)
+    path = str(tmp_path / "1.5.csv")
+    IO.write_dump(path, r, np.full(50, 2), v=v, rho=rho, p=rho - 1000, lossless=True)
+    back = IO.read_dump(path, dim=3)
+    assert np.array_equal(back["r"], r) and np.array_equal(back["v"], v) and np.array_equal(back["rho"], rho)
+    assert P.start_time_from_dump_name(path) == 1.5
+
+
+def test_dump_size_mismatch_is_the_reference_error(tmp_path):
+    path = str(tmp_path / "0.csv")
+    IO.write_dump(path, np.zeros((3, 2)), np.full(3, 2), v=np.zeros((3, 2)), rho=np.ones(3))
+    with pytest.raises(RuntimeError, match="dump corrupted: dump ntotal doesn't match params.ntotal!"):
+        IO.read_dump(path, dim=2, ntotal=4)
+
+
+def test_golden_case_derived_constants():
+    """Input.cpp:93-138 on the golden case: h = delta * intf_hsml_coef, mass =
+    rho0 delta^dim, cubic cell factor 2, c^2 = 200 g d coef (method 0)."""
+    prm, st, _ = load_dambreak_case()
+    assert prm.hsml == pytest.approx(prm.delta * prm.intf_hsml_coef, rel=0, abs=0)
+    assert prm.mass == pytest.approx(prm.rho0 * prm.delta ** prm.dim, rel=1e-15)
+    assert prm.cell_scale_k == 2.0
+    if prm.eos_sound_vel_method == P.EOS_SOUND_VEL_DAM_BREAK:
+        assert prm.eos_c_sqr == pytest.approx(200 * prm.g * prm.depth * prm.eos_sound_vel_coef, rel=1e-15)
