@@ -10,12 +10,17 @@
 // slot order produces (a stable sort orders by the composite (key, slot)).
 //
 //   (integrator)       counts the changed slots per tile of kResortTile
-//   resort_compact     chg_before[s] = changed slots before s; the changed
-//                      slots as 64-bit composites (key << 32 | slot)
+//   resort_compact     block 0 first runs the step prologue; every block
+//                      writes chg_before[s] = changed slots before s and the
+//                      changed slots of its tile as 64-bit composites
+//                      (key << 32 | slot)
 //   resort_sort_small  total C; more than `cap` halts the step with
 //                      kStatusResort (the host re-runs it with the full
 //                      radix sort), else a bitonic sort of the composites in
-//                      LDS (one block)
+//                      LDS (one block).  (Folding it into the compaction as
+//                      a last-block-done step needs a device-scope fence in
+//                      every block, which on the multi-XCD chip writes back
+//                      the L2 and cost 0.23 ms per step.)
 //   resort_scatter     places every slot: unchanged s at s - chg_before[s] +
 //                      the number of changed composites below (key, s);
 //                      changed composite g at g + p - chg_before[p], p = the
@@ -25,6 +30,7 @@
 #pragma once
 
 #include "device_common.hpp"
+#include "step_kernels.hpp"
 
 namespace rrsph {
 
@@ -50,14 +56,43 @@ __device__ __forceinline__ uint32_t resort_block_sum(uint32_t v, uint32_t* s_w) 
     return t;
 }
 
-__global__ void __launch_bounds__(kResortBlock) resort_compact(const uint32_t* __restrict__ key_next,
+// Bitonic sort of chg[0, C) by one block of kResortBlock threads in LDS (C <= kResortCap).
+__device__ __forceinline__ void resort_sort_block(uint32_t C, unsigned long long* __restrict__ chg,
+                                                  unsigned long long* s) {
+    uint32_t P2 = 1;
+    while (P2 < C) P2 <<= 1;
+    for (uint32_t i = threadIdx.x; i < P2; i += kResortBlock) s[i] = i < C ? chg[i] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= P2; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < P2; i += kResortBlock) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const unsigned long long a = s[i], b = s[ixj];
+                    const bool asc = (i & k) == 0;
+                    if ((a > b) == asc) {
+                        s[i] = b;
+                        s[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < C; i += kResortBlock) chg[i] = s[i];
+}
+
+__global__ void __launch_bounds__(kResortBlock) resort_compact(StepParams P, const uint32_t* __restrict__ key_next,
                                                                const uint32_t* __restrict__ skeyA, uint32_t n,
                                                                const uint32_t* __restrict__ tile_cnt, uint32_t cap,
                                                                uint32_t* __restrict__ chg_before,
                                                                unsigned long long* __restrict__ chg,
-                                                               const Ctrl* __restrict__ ctrl) {
-    if (ctrl->halted) return;
+                                                               Ctrl* __restrict__ ctrl) {
     __shared__ uint32_t s_w[kResortBlock / kWave];
+    // the prologue's control-block fields are read by the later kernels of
+    // the step only; the compaction below (scratch arrays) runs whether or
+    // not the step halts
+    if (blockIdx.x == 0 && threadIdx.x == 0) step_prologue_body(P, ctrl);
     const uint32_t tid = threadIdx.x, wave = tid / kWave;
     const uint32_t lane = __lane_id();
     const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -114,27 +149,7 @@ __global__ void __launch_bounds__(kResortBlock) resort_sort_small(const uint32_t
         ctrl->resort_count = C;
         chg_before[n] = C;
     }
-    uint32_t P = 1;
-    while (P < C) P <<= 1;
-    for (uint32_t i = threadIdx.x; i < P; i += kResortBlock) s[i] = i < C ? chg[i] : ~0ull;
-    __syncthreads();
-    for (uint32_t k = 2; k <= P; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < P; i += kResortBlock) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const unsigned long long a = s[i], b = s[ixj];
-                    const bool asc = (i & k) == 0;
-                    if ((a > b) == asc) {
-                        s[i] = b;
-                        s[ixj] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (uint32_t i = threadIdx.x; i < C; i += kResortBlock) chg[i] = s[i];
+    resort_sort_block(C, chg, s);
 }
 
 // Number of t in [0, len) with f(t) < x for a strictly increasing f, found by

@@ -237,13 +237,14 @@ void enqueue_sort(rrsph_ctx* c, Ctrl* ctrl, Launcher& L) {
     c->sorted_perm = src_v;
 }
 
-// Incremental re-sort of key_next against skeyA into kbuf[0] / vbuf[0].
+// Incremental re-sort of key_next against skeyA into kbuf[0] / vbuf[0]; its
+// first kernel also runs the step prologue.
 void enqueue_resort(rrsph_ctx* c, Launcher& L) {
     const uint32_t n = c->P.n;
     const uint32_t nb = grid_blocks(n, kBlock);
     L(K_RESORT_COMPACT, [&] {
-        resort_compact<<<c->rtiles, kResortBlock, 0, c->stream>>>(c->key_next, c->skeyA, n, c->rtile, c->resort_cap,
-                                                                  c->chg_before, c->chg, c->ctrl);
+        resort_compact<<<c->rtiles, kResortBlock, 0, c->stream>>>(c->P, c->key_next, c->skeyA, n, c->rtile,
+                                                                  c->resort_cap, c->chg_before, c->chg, c->ctrl);
     });
     L(K_RESORT_SORT, [&] {
         resort_sort_small<<<1, kResortBlock, 0, c->stream>>>(c->rtile, c->rtiles, c->chg_before, n, c->resort_cap,
@@ -262,9 +263,12 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof, bool full_s
     const StepParams& P = c->P;
     const uint32_t nb = grid_blocks(P.n, kBlock);
     const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
-    L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
-    if (full_sort) enqueue_sort(c, c->ctrl, L);
-    else enqueue_resort(c, L);
+    if (full_sort) {
+        L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+        enqueue_sort(c, c->ctrl, L);
+    } else {
+        enqueue_resort(c, L);
+    }
     StateA<R> A = stateA<R>(c);
     StateB<R> B = stateB<R>(c, c->pred, c->pres);
     L(K_REORDER, [&] {
@@ -316,9 +320,12 @@ template <class R> void enqueue_step3_t(rrsph_ctx* c, Profiler* prof, bool full_
     const StepParams& P = c->P;
     const uint32_t nb = grid_blocks(P.n, kBlock);
     const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
-    L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
-    if (full_sort) enqueue_sort(c, c->ctrl, L);
-    else enqueue_resort(c, L);
+    if (full_sort) {
+        L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+        enqueue_sort(c, c->ctrl, L);
+    } else {
+        enqueue_resort(c, L);
+    }
     StateA3<R> A = stateA3<R>(c);
     StateB3<R> B = stateB3<R>(c, c->pred, c->pw, c->pres);
     L(K_REORDER, [&] {

@@ -149,7 +149,7 @@ __device__ __forceinline__ void slots_max2(const unsigned long long* __restrict_
 }
 
 // One thread: commit the previous step, decide whether to halt, snapshot.
-__global__ void step_prologue(StepParams P, Ctrl* ctrl) {
+__device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ctrl) {
     Ctrl& c = *ctrl;
     if (c.halted) return;
     if (c.pending) {
@@ -194,6 +194,8 @@ __global__ void step_prologue(StepParams P, Ctrl* ctrl) {
     c.mumax_bits = 0;
     c.outside = 0;
 }
+
+__global__ void step_prologue(StepParams P, Ctrl* ctrl) { step_prologue_body(P, ctrl); }
 
 // Cell-start table over keys 0 .. ncells (+1 sentinel entry):
 // LB[c] = first sorted index whose key >= c.  Lane s owns the cells
