@@ -151,12 +151,13 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
     return v;
 }
 
+constexpr uint32_t kCollect3 = 4;  // candidates per batch of the 3D walk
 template <uint32_t kChunk, class F>
 __device__ __forceinline__ void collect_neighbours3(const StepParams& P, bool active, uint32_t cap, uint32_t s,
                                                     uint32_t key, double4 pj, const double4* __restrict__ pos,
                                                     const uint32_t* __restrict__ LB, uint32_t* __restrict__ hits,
                                                     bool& overflow, F&& flush) {
-    static_assert(kChunk >= 2 * kCollect, "hit list too short");
+    static_assert(kChunk >= 2 * kCollect3, "hit list too short");
     const uint32_t cz = key % (uint32_t)P.ncz;
     const uint32_t cxy = key / (uint32_t)P.ncz;
     const int cx = (int)(cxy / (uint32_t)P.ncy);
@@ -173,25 +174,25 @@ __device__ __forceinline__ void collect_neighbours3(const StepParams& P, bool ac
         const uint32_t len = ok ? LB[col + z1 + 1] - b : 0u;
         const uint32_t lmax = wave_max_u32(len);
 #pragma unroll 1
-        for (uint32_t i = 0; i < lmax; i += kCollect) {
-            double4 q[kCollect];
+        for (uint32_t i = 0; i < lmax; i += kCollect3) {
+            double4 q[kCollect3];
 #pragma unroll
-            for (uint32_t u = 0; u < kCollect; ++u) q[u] = pos[len ? b + min(i + u, len - 1) : s];
-            bool hit[kCollect];
+            for (uint32_t u = 0; u < kCollect3; ++u) q[u] = pos[len ? b + min(i + u, len - 1) : s];
+            bool hit[kCollect3];
 #pragma unroll
-            for (uint32_t u = 0; u < kCollect; ++u) {
+            for (uint32_t u = 0; u < kCollect3; ++u) {
                 const double dx = __dsub_rn(q[u].x, pj.x), dy = __dsub_rn(q[u].y, pj.y), dz = __dsub_rn(q[u].z, pj.z);
                 hit[u] = (i + u < len) & (b + i + u != s) & (dist2_exact3(dx, dy, dz) < P.cut2);
             }
-            if (done + nh + kCollect <= cap) {
+            if (done + nh + kCollect3 <= cap) {
 #pragma unroll
-                for (uint32_t u = 0; u < kCollect; ++u) {
+                for (uint32_t u = 0; u < kCollect3; ++u) {
                     hits[nh * kBlock] = b + i + u;
                     nh += hit[u] ? 1u : 0u;
                 }
             } else {
 #pragma unroll
-                for (uint32_t u = 0; u < kCollect; ++u) {
+                for (uint32_t u = 0; u < kCollect3; ++u) {
                     if (!hit[u]) continue;
                     if (done + nh < cap) {
                         hits[nh * kBlock] = b + i + u;
@@ -201,7 +202,7 @@ __device__ __forceinline__ void collect_neighbours3(const StepParams& P, bool ac
                     }
                 }
             }
-            if (__any(nh + kCollect > kChunk)) {  // wave-uniform
+            if (__any(nh + kCollect3 > kChunk)) {  // wave-uniform
                 flush(nh);
                 done += nh;
                 nh = 0;

@@ -402,7 +402,9 @@ __device__ __forceinline__ void for_each_neighbour_batched(const StepParams& P, 
 // body max_lanes(hits) times instead of once per candidate iteration in
 // which any lane hits.  Same neighbour set, visiting order and overflow
 // rule (first `cap` hits kept, the rest flagged) as for_each_neighbour.
-constexpr uint32_t kCollect = 4;
+// Eight candidates per batch: measured 3.5% faster than four at C1-1M FP32
+// (two and twelve are slower, sixteen much slower).
+constexpr uint32_t kCollect = 8;
 template <uint32_t kChunk, class F>
 __device__ __forceinline__ void collect_neighbours(const StepParams& P, uint32_t cap, uint32_t s, uint32_t key,
                                                    double2 pj, const double2* __restrict__ pos,
