@@ -492,7 +492,9 @@ __global__ void __launch_bounds__(kBlock) integrate3_kernel(StepParams P, Ctrl* 
     if (s < P.n) {
         const real4<R> d = der[s];
         const real4<R> x = xs[s];
-        integrate_one3<R>(P, c, s, B.pos[s], B.va[s], B.pred[s], B.oid[s], d.x, d.y, d.z, d.w, x.x, x.y, x.z, dt, A,
+        // the predicted state is needed for the summed density only
+        const real4<R> pr = P.density_treatment != 0 ? real4<R>{} : B.pred[s];
+        integrate_one3<R>(P, c, s, B.pos[s], B.va[s], pr, B.oid[s], d.x, d.y, d.z, d.w, x.x, x.y, x.z, dt, A,
                           key_next, skey[s], outside, bad, changed);
     }
     wave_count_add(&c.outside, outside);

@@ -928,7 +928,9 @@ __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* _
     if (s < P.n) {
         const real4<R> d = der[s];
         const real4<R> x = xs[s];
-        integrate_one<R>(P, c, s, B.pos[s], B.va[s], B.pred[s], B.oid[s], d.x, d.y, d.z, x.x, x.y, dt, A, key_next,
+        // the predicted state is needed for the summed density only
+        const real4<R> pr = P.density_treatment != 0 ? real4<R>{} : B.pred[s];
+        integrate_one<R>(P, c, s, B.pos[s], B.va[s], pr, B.oid[s], d.x, d.y, d.z, x.x, x.y, dt, A, key_next,
                          skey[s], outside, bad, changed);
     }
     wave_count_add(&c.outside, outside);
