@@ -18,6 +18,10 @@ template <class R> struct V4;
 template <> struct V4<float> { using type = float4; };
 template <> struct V4<double> { using type = double4; };
 template <class R> using real4 = typename V4<R>::type;
+template <class R> struct V2;
+template <> struct V2<float> { using type = float2; };
+template <> struct V2<double> { using type = double2; };
+template <class R> using real2 = typename V2<R>::type;
 
 // |r|^2 = dx*dx + dy*dy with every product and sum rounded separately (no
 // fused multiply-add), the arithmetic of the reference's FP64 neighbour

@@ -24,11 +24,6 @@ namespace rrsph {
 constexpr int kTile = kWave;          // particles per tile (one wavefront)
 constexpr int kTilesPerBlock = 4;     // independent tiles (wavefronts) per workgroup
 
-template <class R> struct Vec2;
-template <> struct Vec2<float> { using type = float2; };
-template <> struct Vec2<double> { using type = double2; };
-template <class R> using real2 = typename Vec2<R>::type;
-
 // candidates staged per tile (LDS: 48 B each in FP32, 80 B in FP64)
 template <class R> constexpr int tile_cap() { return sizeof(R) == 4 ? 320 : 256; }
 

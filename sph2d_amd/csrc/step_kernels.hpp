@@ -593,12 +593,10 @@ __device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restr
             d.z = drho;
             d.w = amu;
             der[s] = d;
-            real4<R> x;
+            real2<R> x;  // 2D: xs holds (avx, avy) pairs
             x.x = avx;
             x.y = avy;
-            x.z = R(0);
-            x.w = R(0);
-            xs[s] = x;
+            ((real2<R>*)xs)[s] = x;
         }
     }
     if (FUSED) {
@@ -927,7 +925,7 @@ __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* _
     uint32_t outside = 0, bad = 0, changed = 0;
     if (s < P.n) {
         const real4<R> d = der[s];
-        const real4<R> x = xs[s];
+        const real2<R> x = ((const real2<R>*)xs)[s];
         // the predicted state is needed for the summed density only
         const real4<R> pr = P.density_treatment != 0 ? real4<R>{} : B.pred[s];
         integrate_one<R>(P, c, s, B.pos[s], B.va[s], pr, B.oid[s], d.x, d.y, d.z, x.x, x.y, dt, A, key_next,
@@ -992,7 +990,7 @@ __global__ void fields_kernel(uint32_t n, const uint32_t* __restrict__ oid, cons
     if (s >= n) return;
     const uint32_t o = oid[s];
     const real4<R> d = der[s];
-    const real4<R> x = xs[s];
+    const real2<R> x = ((const real2<R>*)xs)[s];  // 2D: (avx, avy) pairs
     a[2 * o] = (double)d.x;
     a[2 * o + 1] = (double)d.y;
     drho[o] = (double)d.z;
