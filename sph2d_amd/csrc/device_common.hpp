@@ -124,13 +124,10 @@ struct StepParams {
     uint32_t blocks_per_xcd;
     uint32_t ntiles;        // tiles of one wavefront (64 particles) for the tiled pair pass
     uint32_t tile_blocks_per_xcd;
-    // incremental re-sort: per kResortTile slots, the number of particles the
-    // integrator moved to another cell (zeroed by the reorder of each step)
-    uint32_t* resort_cnt;
-    uint32_t resort_tiles;
+    // incremental re-sort: the integrator stores the ballot of the slots of
+    // [64 w, 64 w + 64) whose particle moved to another cell in chg_mask[w]
+    unsigned long long* chg_mask;
 };
-
-constexpr uint32_t kResortTile = 1024;  // slots per re-sort tile (a multiple of kBlock)
 
 // XCD-aware block index: the hardware deals consecutive workgroups to the 8
 // XCDs round-robin; this mapping gives each XCD one contiguous slice of the

@@ -9,24 +9,23 @@
 // which is exactly the permutation the stable LSD radix sort of the keys in
 // slot order produces (a stable sort orders by the composite (key, slot)).
 //
-//   (integrator)       counts the changed slots per tile of kResortTile
-//   resort_compact     block 0 first runs the step prologue; every block
-//                      writes chg_before[s] = changed slots before s and the
-//                      changed slots of its tile as 64-bit composites
-//                      (key << 32 | slot)
-//   resort_sort_small  total C; more than `cap` halts the step with
-//                      kStatusResort (the host re-runs it with the full
-//                      radix sort), else a bitonic sort of the composites in
-//                      LDS (one block).  (Folding it into the compaction as
-//                      a last-block-done step needs a device-scope fence in
-//                      every block, which on the multi-XCD chip writes back
-//                      the L2 and cost 0.23 ms per step.)
-//   resort_scatter     places every slot: unchanged s at s - chg_before[s] +
-//                      the number of changed composites below (key, s);
-//                      changed composite g at g + p - chg_before[p], p = the
-//                      number of slots whose (skeyA[t], t) is below it
-// Every position written is in [0, n) whatever the keys hold: s - chg_before[s]
-// counts unchanged slots and g counts changed ones.
+//   (integrator)       stores each wavefront's ballot of changed slots
+//   resort_sort_small  one block: the step prologue; in passes of 16384
+//                      wavefronts, the exclusive prefix wpre[w] of the
+//                      changed counts of the wavefronts before w and the
+//                      changed slots gathered as 64-bit composites
+//                      (key << 32 | slot) in slot order; a total C above `cap`
+//                      halts the step with kStatusResort (the host re-runs it
+//                      with the full radix sort), else a bitonic sort of the
+//                      composites in LDS
+//   resort_scatter     places every slot: unchanged s at s - cb(s) + the
+//                      number of changed composites below (skeyA[s], s);
+//                      changed composite g at g + p - cb(p), p = the number
+//                      of slots whose (skeyA[t], t) is below it; cb(s) = the
+//                      changed slots below s = wpre[s / 64] + the set bits of
+//                      the ballot below lane s % 64
+// Every position written is in [0, n) whatever the keys hold: s - cb(s)
+// counts unchanged slots and g changed ones, both from the same ballots.
 #pragma once
 
 #include "device_common.hpp"
@@ -35,7 +34,6 @@
 namespace rrsph {
 
 constexpr uint32_t kResortBlock = 1024;
-constexpr uint32_t kResortItems = kResortTile / kResortBlock;
 constexpr uint32_t kResortCap = 8192;           // changed slots sorted in LDS (64 KiB)
 constexpr uint32_t kStatusResort = 100;         // internal: re-run this step with the full sort
 
@@ -43,17 +41,32 @@ __device__ __forceinline__ unsigned long long resort_comp(uint32_t key, uint32_t
     return ((unsigned long long)key << 32) | slot;
 }
 
-// Sum of v over a block of kResortBlock threads (result in every thread).
-__device__ __forceinline__ uint32_t resort_block_sum(uint32_t v, uint32_t* s_w) {
+__device__ __forceinline__ unsigned long long lanes_below(uint32_t lane) {
+    return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// Exclusive prefix of v over a block of kResortBlock threads; *total gets
+// the sum (in every thread).
+__device__ __forceinline__ uint32_t resort_block_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / kWave;
+    uint32_t incl = v;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    if (__lane_id() == 0) s_w[threadIdx.x / kWave] = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t u = __shfl_up(incl, off);
+        if (lane >= (uint32_t)off) incl += u;
+    }
+    if (lane == kWave - 1) s_w[wave] = incl;
     __syncthreads();
-    uint32_t t = 0;
+    uint32_t before = 0, t = 0;
 #pragma unroll
-    for (uint32_t w = 0; w < kResortBlock / kWave; ++w) t += s_w[w];
+    for (uint32_t w = 0; w < kResortBlock / kWave; ++w) {
+        const uint32_t sw = s_w[w];
+        if (w < wave) before += sw;
+        t += sw;
+    }
     __syncthreads();
-    return t;
+    *total = t;
+    return before + incl - v;
 }
 
 // Bitonic sort of chg[0, C) by one block of kResortBlock threads in LDS (C <= kResortCap).
@@ -82,61 +95,77 @@ __device__ __forceinline__ void resort_sort_block(uint32_t C, unsigned long long
     for (uint32_t i = threadIdx.x; i < C; i += kResortBlock) chg[i] = s[i];
 }
 
-__global__ void __launch_bounds__(kResortBlock) resort_compact(StepParams P, const uint32_t* __restrict__ key_next,
-                                                               const uint32_t* __restrict__ skeyA, uint32_t n,
-                                                               const uint32_t* __restrict__ tile_cnt, uint32_t cap,
-                                                               uint32_t* __restrict__ chg_before,
-                                                               unsigned long long* __restrict__ chg,
-                                                               Ctrl* __restrict__ ctrl) {
-    __shared__ uint32_t s_w[kResortBlock / kWave];
-    // the prologue's control-block fields are read by the later kernels of
-    // the step only; the compaction below (scratch arrays) runs whether or
-    // not the step halts
-    if (blockIdx.x == 0 && threadIdx.x == 0) step_prologue_body(P, ctrl);
-    const uint32_t tid = threadIdx.x, wave = tid / kWave;
-    const uint32_t lane = __lane_id();
-    const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // changed slots in the tiles before this one
-    uint32_t before = 0;
-    for (uint32_t t = tid; t < blockIdx.x; t += kResortBlock) before += tile_cnt[t];
-    uint32_t run = resort_block_sum(before, s_w);
-    const uint32_t base = blockIdx.x * kResortTile;
-    for (uint32_t it = 0; it < kResortItems; ++it) {
-        const uint32_t s = base + it * kResortBlock + tid;
-        const bool live = s < n;
-        const uint32_t k = live ? key_next[s] : 0u;
-        const bool changed = live && k != skeyA[s];
-        const unsigned long long m = __ballot(changed);
-        if (lane == 0) s_w[wave] = (uint32_t)__popcll(m);
-        __syncthreads();
-        uint32_t wbase = 0, total = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < kResortBlock / kWave; ++w) {
-            const uint32_t sw = s_w[w];
-            if (w < wave) wbase += sw;
-            total += sw;
-        }
-        const uint32_t c = run + wbase + (uint32_t)__popcll(m & lt);
-        if (live) chg_before[s] = c;
-        if (changed && c < cap) chg[c] = resort_comp(k, s);
-        run += total;
-        __syncthreads();
-    }
-}
+constexpr uint32_t kResortChunk = 16;  // ballots per thread and pass of the scan
+// counts of one pass in LDS, one padding word after every 64 so that both the
+// coalesced (j * kResortBlock + t) and the per-thread (t * kResortChunk + j)
+// accesses of a wave hit 64 different banks
+constexpr uint32_t kResortPass = kResortBlock * kResortChunk;
+__device__ __forceinline__ uint32_t resort_pad(uint32_t i) { return i + (i >> 6); }
+constexpr uint32_t kResortLds = kResortCap + kResortPass / 128;  // 64-bit words: sort buffer or padded counts
+static_assert(kResortPass + kResortPass / 64 <= 2 * kResortLds, "a pass of counts fits the sort buffer");
 
-__global__ void __launch_bounds__(kResortBlock) resort_sort_small(const uint32_t* __restrict__ tile_cnt,
-                                                                  uint32_t ntiles, uint32_t* __restrict__ chg_before,
-                                                                  uint32_t n, uint32_t cap,
-                                                                  unsigned long long* __restrict__ chg,
+// One block.  Each pass loads kResortChunk ballots per thread (coalesced),
+// puts their counts in LDS, scans them (thread t owns the kResortChunk
+// consecutive counts from t * kResortChunk), writes wpre coalesced and
+// gathers the changed slots' composites to chg[wpre[w] + j] (j-th set bit).
+__global__ void __launch_bounds__(kResortBlock) resort_sort_small(StepParams P, const uint32_t* __restrict__ key_next,
+                                                                  uint32_t cap, unsigned long long* __restrict__ chg,
+                                                                  uint32_t* __restrict__ wpre,
                                                                   Ctrl* __restrict__ ctrl) {
-    if (ctrl->halted) return;
     __shared__ uint32_t s_w[kResortBlock / kWave];
-    __shared__ unsigned long long s[kResortCap];
-    uint32_t part = 0;
-    for (uint32_t t = threadIdx.x; t < ntiles; t += kResortBlock) part += tile_cnt[t];
-    const uint32_t C = resort_block_sum(part, s_w);
+    __shared__ unsigned long long s[kResortLds];
+    uint32_t* s32 = (uint32_t*)s;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nw = (P.n + kWave - 1) / kWave;
+    const unsigned long long* __restrict__ mask = P.chg_mask;
+    unsigned long long m[kResortChunk];
+    auto load_pass = [&](uint32_t base) {
+#pragma unroll
+        for (uint32_t j = 0; j < kResortChunk; ++j) {
+            const uint32_t w = base + j * kResortBlock + tid;
+            m[j] = w < nw ? mask[w] : 0ull;
+        }
+    };
+    load_pass(0);  // in flight during the prologue
+    if (tid == 0) step_prologue_body(P, ctrl);  // published by the barriers below
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < nw; base += kResortPass) {
+        if (base != 0) load_pass(base);
+#pragma unroll
+        for (uint32_t j = 0; j < kResortChunk; ++j) s32[resort_pad(j * kResortBlock + tid)] = (uint32_t)__popcll(m[j]);
+        __syncthreads();
+        uint32_t part = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < kResortChunk; ++j) part += s32[resort_pad(tid * kResortChunk + j)];
+        uint32_t total;
+        uint32_t pre = run + resort_block_scan(part, s_w, &total);
+#pragma unroll
+        for (uint32_t j = 0; j < kResortChunk; ++j) {  // this thread's own entries
+            const uint32_t c = s32[resort_pad(tid * kResortChunk + j)];
+            s32[resort_pad(tid * kResortChunk + j)] = pre;
+            pre += c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kResortChunk; ++j) {
+            const uint32_t w = base + j * kResortBlock + tid;
+            if (w < nw) {
+                uint32_t p = s32[resort_pad(j * kResortBlock + tid)];
+                wpre[w] = p;
+                for (unsigned long long mm = m[j]; mm != 0; mm &= mm - 1, ++p) {
+                    const uint32_t sl = w * kWave + (uint32_t)(__ffsll((long long)mm) - 1);
+                    if (p < cap) chg[p] = resort_comp(key_next[sl], sl);
+                }
+            }
+        }
+        run += total;
+        __syncthreads();  // s32 is refilled by the next pass; chg is read by the sort
+    }
+    __syncthreads();
+    if (ctrl->halted) return;
+    const uint32_t C = run;
     if (C > cap) {
-        if (threadIdx.x == 0) {
+        if (tid == 0) {
             Ctrl& c = *ctrl;
             c.resort_count = C;
             c.halted = 1;
@@ -145,10 +174,7 @@ __global__ void __launch_bounds__(kResortBlock) resort_sort_small(const uint32_t
         }
         return;
     }
-    if (threadIdx.x == 0) {
-        ctrl->resort_count = C;
-        chg_before[n] = C;
-    }
+    if (tid == 0) ctrl->resort_count = C;
     resort_sort_block(C, chg, s);
 }
 
@@ -171,17 +197,17 @@ __device__ __forceinline__ uint32_t wave_count_below(F&& f, uint32_t len, unsign
     return lo;
 }
 
-__global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restrict__ key_next,
-                                                         const uint32_t* __restrict__ skeyA,
-                                                         const uint32_t* __restrict__ chg_before,
-                                                         const unsigned long long* __restrict__ chg, uint32_t n,
+__global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restrict__ skeyA,
+                                                         const unsigned long long* __restrict__ chg,
+                                                         const unsigned long long* __restrict__ mask,
+                                                         const uint32_t* __restrict__ wpre, uint32_t n,
                                                          uint32_t* __restrict__ skeyB, uint32_t* __restrict__ perm,
                                                          const Ctrl* __restrict__ ctrl) {
     if (ctrl->halted) return;
     __shared__ uint32_t s_lo, s_hi;
     __shared__ unsigned long long s_c[kBlock];
     const uint32_t C = ctrl->resort_count;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = __lane_id();
     const uint32_t s0 = blockIdx.x * kBlock;
     const uint32_t s = s0 + tid;
     auto changed_at = [&](uint32_t t) { return chg[t]; };
@@ -201,8 +227,9 @@ __global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restr
     if (staged && tid < cnt) s_c[tid] = chg[lo + tid];
     __syncthreads();
     if (s < n) {
-        const uint32_t k = skeyA[s];
-        if (key_next[s] == k) {
+        const unsigned long long m = mask[s / kWave];
+        if (!((m >> lane) & 1ull)) {
+            const uint32_t k = skeyA[s];
             const unsigned long long x = resort_comp(k, s);
             uint32_t r = lo;
             if (staged) {
@@ -216,7 +243,8 @@ __global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restr
                 }
                 r += a;
             }
-            const uint32_t pos = s - chg_before[s] + r;
+            const uint32_t cb = wpre[s / kWave] + (uint32_t)__popcll(m & lanes_below(lane));
+            const uint32_t pos = s - cb + r;
             skeyB[pos] = k;
             perm[pos] = s;
         }
@@ -228,8 +256,10 @@ __global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restr
     for (uint32_t g = blockIdx.x * (kBlock / kWave) + tid / kWave; g < C; g += waves) {
         const unsigned long long x = chg[g];
         const uint32_t p = wave_count_below(slot_at, n, x);
-        if (__lane_id() == 0) {
-            const uint32_t pos = g + p - chg_before[p];
+        if (lane == 0) {
+            const uint32_t cb = p < n ? wpre[p / kWave] + (uint32_t)__popcll(mask[p / kWave] & lanes_below(p % kWave))
+                                      : C;
+            const uint32_t pos = g + p - cb;
             skeyB[pos] = (uint32_t)(x >> 32);
             perm[pos] = (uint32_t)x;
         }

@@ -37,15 +37,13 @@ enum KernelClass : uint32_t {
     K_DENSITY_SUM,
     K_PAIR,
     K_INTEGRATE,
-    K_RESORT_COMPACT,
     K_RESORT_SORT,
     K_RESORT_SCATTER,
     K_NUM_CLASSES
 };
 const char* const kClassNames[K_NUM_CLASSES] = {"step_prologue", "radix_upsweep", "radix_scan", "radix_downsweep",
                                                 "reorder_kernel", "density_sum_kernel", "pair_kernel",
-                                                "integrate_kernel", "resort_compact", "resort_sort_small",
-                                                "resort_scatter"};
+                                                "integrate_kernel", "resort_sort_small", "resort_scatter"};
 
 struct Profiler {
     std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> rec;
@@ -142,10 +140,9 @@ struct rrsph_ctx {
     const uint32_t* sorted_perm = nullptr;
     // incremental re-sort
     uint32_t* skeyA = nullptr;          // key each slot of A was sorted by
-    uint32_t* chg_before = nullptr;     // changed slots before s (n + 1 entries)
-    uint32_t* rtile = nullptr;          // per-tile changed counts (filled by the integrator)
-    unsigned long long* chg = nullptr;  // changed slots as (key << 32 | slot)
-    uint32_t rtiles = 0;
+    unsigned long long* chg = nullptr;       // changed slots as (key << 32 | slot)
+    unsigned long long* chg_mask = nullptr;  // per wavefront of slots: ballot of the changed ones
+    uint32_t* wpre = nullptr;                // per wavefront: changed slots in the wavefronts before
     uint32_t resort_cap = kResortCap;
     bool incremental = true;            // false: every step runs the full radix sort
     bool need_full = true;              // skeyA does not describe A (after upload)
@@ -242,16 +239,12 @@ void enqueue_sort(rrsph_ctx* c, Ctrl* ctrl, Launcher& L) {
 void enqueue_resort(rrsph_ctx* c, Launcher& L) {
     const uint32_t n = c->P.n;
     const uint32_t nb = grid_blocks(n, kBlock);
-    L(K_RESORT_COMPACT, [&] {
-        resort_compact<<<c->rtiles, kResortBlock, 0, c->stream>>>(c->P, c->key_next, c->skeyA, n, c->rtile,
-                                                                  c->resort_cap, c->chg_before, c->chg, c->ctrl);
-    });
     L(K_RESORT_SORT, [&] {
-        resort_sort_small<<<1, kResortBlock, 0, c->stream>>>(c->rtile, c->rtiles, c->chg_before, n, c->resort_cap,
-                                                             c->chg, c->ctrl);
+        resort_sort_small<<<1, kResortBlock, 0, c->stream>>>(c->P, c->key_next, c->resort_cap, c->chg, c->wpre,
+                                                             c->ctrl);
     });
     L(K_RESORT_SCATTER, [&] {
-        resort_scatter<<<nb, kBlock, 0, c->stream>>>(c->key_next, c->skeyA, c->chg_before, c->chg, n, c->kbuf[0],
+        resort_scatter<<<nb, kBlock, 0, c->stream>>>(c->skeyA, c->chg, c->chg_mask, c->wpre, n, c->kbuf[0],
                                                      c->vbuf[0], c->ctrl);
     });
     c->sorted_keys = c->kbuf[0];
@@ -571,7 +564,6 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     if (ctx->passes == 0) ctx->passes = 1;
     ctx->ntiles = grid_blocks(P.n, kSortTile);
     ctx->fp32 = p.precision == RRSPH_FP32;
-    ctx->rtiles = grid_blocks(P.n, kResortTile);
     if (const char* e = std::getenv("RRSPH_HIP_SORT")) ctx->incremental = std::strcmp(e, "full") != 0;
     if (const char* e = std::getenv("RRSPH_HIP_RESORT_CAP")) {
         const long v = std::strtol(e, nullptr, 10);
@@ -613,7 +605,7 @@ void free_all(rrsph_ctx* c) {
                     c->oidB, c->key_next, c->kbuf[0], c->kbuf[1], c->vbuf[0], c->vbuf[1], c->hist, c->digit_total,
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
-                    c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg_before, c->rtile, c->chg, c->pw,
+                    c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
                     c->pw_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -702,11 +694,10 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->util_ctrl = dalloc<Ctrl>(1, e);
     ctx->red = dalloc<double>(kDiagSlots * (size_t)kDiagMaxBlocks + 64, e);
     ctx->skeyA = dalloc<uint32_t>(n, e);
-    ctx->chg_before = dalloc<uint32_t>(n + 1, e);
-    ctx->rtile = dalloc<uint32_t>(ctx->rtiles + 1, e);
-    ctx->P.resort_cnt = ctx->rtile;
-    ctx->P.resort_tiles = ctx->rtiles;
     ctx->chg = dalloc<unsigned long long>(kResortCap, e);
+    ctx->chg_mask = dalloc<unsigned long long>(n / kWave + 1, e);
+    ctx->wpre = dalloc<uint32_t>(n / kWave + 1, e);
+    ctx->P.chg_mask = ctx->chg_mask;
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     std::memset(ctx->host_ctrl, 0, sizeof(Ctrl));

@@ -64,8 +64,6 @@ __global__ void __launch_bounds__(kBlock) reorder3_kernel(StepParams P, const Ct
     if (blockIdx.x == 0) {
         maxslots[threadIdx.x] = 0;
         maxslots[kMaxSlots + threadIdx.x] = 0;
-        if (skey_keep)
-            for (uint32_t t = threadIdx.x; t < P.resort_tiles; t += kBlock) P.resort_cnt[t] = 0;
     }
     const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
     const bool live = s0 < P.n;
@@ -456,7 +454,7 @@ __global__ void __launch_bounds__(kBlock) pair3_kernel(StepParams P, Ctrl* __res
                               key_next, skey[s], outside, bad, changed);
         }
         wave_count_add(&ctrl->outside, outside);
-        wave_count_add(&P.resort_cnt[s / kResortTile], changed);
+        store_changed_mask(P, changed, s);
         if (bad) atomicOr(&ctrl->grid_overflow, 1u);
     }
 }
@@ -498,7 +496,7 @@ __global__ void __launch_bounds__(kBlock) integrate3_kernel(StepParams P, Ctrl* 
                           key_next, skey[s], outside, bad, changed);
     }
     wave_count_add(&c.outside, outside);
-    wave_count_add(&P.resort_cnt[s / kResortTile], changed);
+    store_changed_mask(P, changed, s);
     if (bad) atomicOr(&c.grid_overflow, 1u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.dt_new = dt;

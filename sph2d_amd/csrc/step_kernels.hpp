@@ -243,9 +243,6 @@ __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctr
     if (blockIdx.x == 0) {  // clear the max slots of this step's reductions
         maxslots[threadIdx.x] = 0;
         maxslots[kMaxSlots + threadIdx.x] = 0;
-        // and the changed-cell counts the integrator of this step fills
-        if (skey_keep)
-            for (uint32_t t = threadIdx.x; t < P.resort_tiles; t += kBlock) P.resort_cnt[t] = 0;
     }
     const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
     const bool live = s0 < P.n;
@@ -561,6 +558,14 @@ __device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c
     changed = key != key_old ? 1u : 0u;
 }
 
+// Store the ballot of the lanes whose particle changed cell, for the next
+// step's incremental re-sort.  Every lane of the wave calls it, lane l with
+// slot s = 64 w + l; every wave with slots below n stores its word.
+__device__ __forceinline__ void store_changed_mask(const StepParams& P, uint32_t changed, uint32_t s) {
+    const unsigned long long m = __ballot(changed != 0);
+    if (__lane_id() == 0 && s < P.n) P.chg_mask[s / kWave] = m;
+}
+
 __device__ __forceinline__ void wave_count_add(uint32_t* dst, uint32_t v) {
     // one atomic per wave
     const unsigned long long any = __ballot(v != 0);
@@ -611,8 +616,7 @@ __device__ __forceinline__ void pair_epilogue(const StepParams& P, Ctrl* __restr
                              key_old, outside, bad, changed);
         }
         wave_count_add(&ctrl->outside, outside);
-        // a wave's slots lie in one re-sort tile; lanes past n count nothing
-        wave_count_add(&P.resort_cnt[s / kResortTile], changed);
+        store_changed_mask(P, changed, s);  // lanes past n did not change
         if (bad) atomicOr(&ctrl->grid_overflow, 1u);
     }
 }
@@ -932,7 +936,7 @@ __global__ void __launch_bounds__(kBlock) integrate_kernel(StepParams P, Ctrl* _
                          skey[s], outside, bad, changed);
     }
     wave_count_add(&c.outside, outside);
-    wave_count_add(&P.resort_cnt[s / kResortTile], changed);
+    store_changed_mask(P, changed, s);
     if (bad) atomicOr(&c.grid_overflow, 1u);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         c.dt_new = dt;
