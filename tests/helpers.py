@@ -110,3 +110,74 @@ def smooth_gauges_np(r, itype, xs, radius, tau, y_ref, ptype=2, zs=None):
         if ws > 0:
             out[g] = y_ref + tau * np.log((w * np.exp((y - y_ref) / tau)).sum() / ws)
     return out
+
+
+def corner_cutoff_case(dim=2, seed=11, per_centre=48, stride=3):
+    """Particles across the corners of their cells at the cut-off.  In every
+    stride-th cell a centre sits at distance rho from a corner of its cell
+    (in 3D a corner or, with one axis free, an edge), rho = cut (1 + e) with
+    e from -1e-3 to +1e-10, and per_centre satellites lie at distance
+    cut (1 + k ulp), |k| <= 8, in directions near the corner's.  With e < 0
+    the satellites sit in the cell diagonally across the corner and are
+    neighbours or not by the last bits of the FP64 test; with e > 0 that
+    cell holds no neighbour.  A walk that skips corner cells or columns
+    (GridFind.cpp:144-176 visits all 3^dim cells) must keep every neighbour."""
+    import copy
+    import itertools
+
+    from sph2d_amd import generators as G
+
+    lay = G.dam_break() if dim == 2 else G.dam_break_3d()
+    prm = copy.copy(lay.params)
+    cut = prm.cell_scale_k * prm.hsml
+    lo = np.array([prm.x_mingeom, prm.y_mingeom, prm.z_mingeom][:dim])
+    hi = np.array([prm.x_maxgeom, prm.y_maxgeom, prm.z_maxgeom][:dim])
+    ncell = np.floor((hi - lo) / cut).astype(int) + 1
+    rng = np.random.default_rng(seed)
+    offsets = [-1e-3, -1e-6, -1e-10, -1e-12, -1e-14, -4.4e-16, 0.0, 4.4e-16, 1e-14, 1e-12, 1e-11, 1e-10]
+    pts = []
+    for cidx in itertools.product(*[range(1, n - 2, stride) for n in ncell]):
+        origin = lo + np.array(cidx) * cut
+        naxes = dim if dim == 2 else int(rng.integers(2, 4))
+        axes = rng.permutation(dim)[:naxes]
+        s = rng.choice([-1.0, 1.0], dim)
+        dirv = np.zeros(dim)
+        dirv[axes] = rng.uniform(0.25, 1.0, naxes)
+        dirv /= np.linalg.norm(dirv)
+        rho = cut * (1.0 + offsets[int(rng.integers(len(offsets)))])
+        corner = origin + np.where(s > 0, cut, 0.0)
+        centre = corner - s * rho * dirv
+        free = np.setdiff1d(np.arange(dim), axes)
+        centre[free] = origin[free] + rng.uniform(0.3, 0.7, len(free)) * cut
+        pts.append(centre)
+        for _ in range(per_centre):
+            d = dirv + rng.normal(0.0, rng.choice([1e-6, 1e-4, 1e-2]), dim)
+            d /= np.linalg.norm(d)
+            r = cut * (1.0 + int(rng.integers(-8, 9)) * 2.220446049250313e-16)
+            pts.append(centre + s * r * d)
+    r = np.array(pts, dtype=np.float64)
+    n = len(r)
+    prm.ntotal = prm.nfluid = n
+    prm.nvirt = 0
+    prm.maxn = 1 << int(np.ceil(np.log2(n)) + 1)
+    prm.max_neighbours = 128
+    itype = np.full(n, 2, dtype=np.int32)
+    return prm, r, itype
+
+
+def corner_pairs_at_cutoff(prm, r, sets, rel=1e-12):
+    """Neighbour pairs whose cells differ along two or more axes (diagonal
+    across a corner or an edge) at distance within rel of the cut-off."""
+    dim = r.shape[1]
+    cut = prm.cell_scale_k * prm.hsml
+    lo = np.array([prm.x_mingeom, prm.y_mingeom, prm.z_mingeom][:dim])
+    key = np.floor((r - lo) / cut).astype(np.int64)
+    count = 0
+    for j, s in enumerate(sets):
+        if len(s) == 0:
+            continue
+        s = np.asarray(s, dtype=np.int64)
+        d = np.sqrt(np.sum((r[s] - r[j]) ** 2, axis=1))
+        diag = np.sum(key[s] != key[j], axis=1) >= 2
+        count += int(np.sum(diag & (np.abs(d - cut) <= rel * cut)))
+    return count

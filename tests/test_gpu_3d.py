@@ -40,6 +40,26 @@ def test_3d_keys_and_neighbour_sets(precision):
     gpu.close()
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_3d_neighbour_sets_across_cell_corners_and_edges(precision):
+    """Centres at the cut-off (to within ulps .. 1e-3) from a corner or an
+    edge of their cell and satellites in the cells across it, at the cut-off
+    to a few ulps: the GPU walk finds exactly the oracle's neighbours there."""
+    from helpers import corner_cutoff_case, corner_pairs_at_cutoff
+
+    prm, r, itype = corner_cutoff_case(dim=3)
+    print("3D corner case n", prm.ntotal, "precision", precision, flush=True)
+    gpu = H.HipStep(prm, precision=precision)
+    gpu.upload(r, np.zeros_like(r), np.full(prm.ntotal, prm.rho0), itype)
+    nb_gpu = gpu.neighbours()
+    gpu.close()
+    st, nb_ref = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    sg, sr = oracle.neighbour_sets(nb_gpu, prm.ntotal), oracle.neighbour_sets(nb_ref, prm.ntotal)
+    assert corner_pairs_at_cutoff(prm, r, sr) > 500
+    assert all(np.array_equal(a, b) for a, b in zip(sg, sr))
+
+
 @pytest.mark.parametrize("density_treatment", [0, 1])
 def test_3d_step_fp64_matches_oracle(density_treatment):
     """3D dam break (summation or continuity density), FP64: fields after one

@@ -228,6 +228,26 @@ def test_neighbour_sets_near_cutoff(precision):
     gpu.close()
 
 
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_neighbour_sets_across_cell_corners(precision):
+    """Centres at the cut-off (to within ulps .. 1e-3) from a corner of their
+    cell and satellites in the cell diagonally across it, at the cut-off to a
+    few ulps: the GPU walk finds exactly the oracle's neighbours there."""
+    from helpers import corner_cutoff_case, corner_pairs_at_cutoff
+
+    prm, r, itype = corner_cutoff_case(dim=2)
+    print("corner case n", prm.ntotal, "precision", precision, flush=True)
+    gpu = H.HipStep(prm, precision=precision)
+    gpu.upload(r, np.zeros_like(r), np.full(prm.ntotal, prm.rho0), itype)
+    nb_gpu = gpu.neighbours()
+    gpu.close()
+    st, nb_ref = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    sg, sr = _sets(nb_gpu, prm.ntotal), _sets(nb_ref, prm.ntotal)
+    assert corner_pairs_at_cutoff(prm, r, sr) > 1000  # the case reaches the trimmed cells
+    assert all(np.array_equal(a, b) for a, b in zip(sg, sr))
+
+
 def _run_with_env(monkeypatch, env, prm, state, nsteps, precision):
     for k, v in env.items():
         monkeypatch.setenv(k, v)
