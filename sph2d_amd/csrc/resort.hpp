@@ -152,9 +152,10 @@ __global__ void __launch_bounds__(kResortBlock) resort_sort_small(StepParams P, 
             if (w < nw) {
                 uint32_t p = s32[resort_pad(j * kResortBlock + tid)];
                 wpre[w] = p;
-                for (unsigned long long mm = m[j]; mm != 0; mm &= mm - 1, ++p) {
+                // past cap the step overflows anyway: gather no further
+                for (unsigned long long mm = m[j]; mm != 0 && p < cap; mm &= mm - 1, ++p) {
                     const uint32_t sl = w * kWave + (uint32_t)(__ffsll((long long)mm) - 1);
-                    if (p < cap) chg[p] = resort_comp(key_next[sl], sl);
+                    chg[p] = resort_comp(key_next[sl], sl);
                 }
             }
         }
