@@ -227,7 +227,7 @@ __global__ void __launch_bounds__(kBlock) density_sum3_kernel(StepParams P, Ctrl
     if ((int)pj.w != 0) {
         for_each_neighbour3(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
                             [&](uint32_t, double dx, double dy, double dz) {
-                                const R q = (R)sqrt(dx * dx + dy * dy + dz * dz) * inv_h;
+                                const R q = pair_dist<R>(dx, dy, dz) * inv_h;
                                 const R t = R(2) - q;
                                 rho += m * (q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q)
                                                       : w_q2 * t * t * t);

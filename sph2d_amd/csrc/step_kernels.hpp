@@ -226,6 +226,19 @@ __device__ __forceinline__ void fill_cell_starts(const StepParams& P, uint32_t s
     }
 }
 
+// |r| of a neighbour pair in the pass precision.  FP64 takes the square root
+// of the FP64 sum; FP32 rounds the differences to float and uses the
+// hardware square root (1 ulp), the FP64 root being a long instruction
+// sequence paid on every neighbour of the summation-density passes.
+template <class R> __device__ __forceinline__ R pair_dist(double dx, double dy, double dz);
+template <> __device__ __forceinline__ double pair_dist<double>(double dx, double dy, double dz) {
+    return sqrt(dx * dx + dy * dy + dz * dz);
+}
+template <> __device__ __forceinline__ float pair_dist<float>(double dx, double dy, double dz) {
+    const float x = (float)dx, y = (float)dy, z = (float)dz;
+    return __builtin_amdgcn_sqrtf(x * x + y * y + z * z);
+}
+
 // ------------------------------------------------------------------ reorder
 // Gather the state into sorted order, predict (PredictHalfStep.cpp:16-22),
 // compute the pressure of the current density (Density.h:112) and fill the
@@ -482,7 +495,7 @@ __global__ void __launch_bounds__(kBlock) density_sum_kernel(StepParams P, Ctrl*
         const double2 pj = B.pos[s];
         for_each_neighbour<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
                                   [&](uint32_t, double dx, double dy) {
-                                      rho += m * kernel_w<R>(K, P.skf_density, (R)sqrt(dx * dx + dy * dy));
+                                      rho += m * kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0));
                                   });
     }
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
