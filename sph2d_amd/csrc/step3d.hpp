@@ -395,8 +395,11 @@ __global__ void __launch_bounds__(kBlock) pair3_kernel(StepParams P, Ctrl* __res
             const uint32_t last = nh - 1;
             uint32_t i1 = my[min(1u, last) * kBlock];
             const uint32_t i0 = my[0];
+            // the current velocity enters the continuity term only; with
+            // summation density its gather is skipped (a uniform branch)
             double4 qn = B.pos[i0];
-            real4<R> van = B.va[i0], prn = B.pred[i0];
+            real4<R> van{}, prn = B.pred[i0];
+            if (K.cont) van = B.va[i0];
             R pwn = B.pw[i0];
 #pragma unroll 1
             for (uint32_t u = 0; u < nh; ++u) {
@@ -404,7 +407,7 @@ __global__ void __launch_bounds__(kBlock) pair3_kernel(StepParams P, Ctrl* __res
                 const real4<R> vai = van, pri = prn;
                 const R pwi = pwn;
                 qn = B.pos[i1];
-                van = B.va[i1];
+                if (K.cont) van = B.va[i1];
                 prn = B.pred[i1];
                 pwn = B.pw[i1];
                 i1 = my[min(u + 2, last) * kBlock];
