@@ -197,6 +197,27 @@ rrsph_status rrsph_hip_diagnostics(rrsph_ctx* ctx, rrsph_diagnostics* out);
 rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t ngauges, double search_n,
                               int32_t ptype, double* out);
 
+/* SPH interpolation of a field at points, FuncAtPoint
+ * (tools/FuncAtPoint/lib/src/fatp.cpp:14-58 findValue, :121-144 calculate):
+ * for each point x, sum_i f_i W(|r_i - x|) m / rho_i over the particles of
+ * the 3^dim cells around x's cell with |r_i - x|^2 < (k h)^2, W the cubic
+ * spline, rho_i the summation density of the current positions
+ * (Density.h:25-61), f the field `field` as rrsph_hip_download returns it.
+ * points holds npoints dim-tuples inside the geometry box (inclusive), at
+ * most RRSPH_MAX_PROBES; out gets npoints values.  Errors as check_input
+ * (fatp.cpp:92-119): "wrong target value", "point is outside bounds"; a
+ * neighbour overflow with consistency_check on is the reference's
+ * RRSPH_ERR_TOO_MANY_NEIGHBOURS.  A time series is one call per saved
+ * time, as calculate() runs one per layer. */
+#define RRSPH_MAX_PROBES 2048
+#define RRSPH_PROBE_VX  0u
+#define RRSPH_PROBE_VY  1u
+#define RRSPH_PROBE_VZ  2u
+#define RRSPH_PROBE_P   3u
+#define RRSPH_PROBE_RHO 4u
+rrsph_status rrsph_hip_probe(rrsph_ctx* ctx, const double* points, uint32_t npoints, uint32_t field,
+                             double* out);
+
 /* ---- differentiation ------------------------------------------------- */
 /* No reference counterpart: the reference has no derivative of its step.
  * The adjoint differentiates the step, 2D or 3D, in FP64 arithmetic on the

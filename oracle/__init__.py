@@ -88,6 +88,8 @@ def lib() -> ctypes.CDLL:
             "oracle_neighbouring_cells": (None, [P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
             "oracle_count_cells": (ctypes.c_uint64, [P, c_int32_p]),
             "oracle_grid_find": (ctypes.c_int32, [P, c_double_p, c_int32_p, c_uint32_p]),
+            "oracle_probe": (ctypes.c_int32, [P, c_double_p, c_int32_p, c_double_p, ctypes.c_uint32, ctypes.c_uint32,
+                                              c_double_p, ctypes.c_uint32, c_double_p]),
             "oracle_density_sum": (None, [P, c_double_p, c_uint32_p, c_double_p, c_double_p]),
             "oracle_density_con": (None, [P, c_double_p, c_double_p, c_uint32_p, c_double_p, c_double_p,
                                           c_double_p]),
@@ -218,6 +220,25 @@ def grid_find(params, r, itype):
     itype = np.ascontiguousarray(itype, dtype=np.int32)
     st = L.oracle_grid_find(ctypes.byref(op), ptr(r), ptr(itype, ctypes.c_int32), ptr(nb, ctypes.c_uint32))
     return st, nb.reshape(params.ntotal, m)
+
+
+PROBE_FIELDS = {"vx": ("v", 0), "vy": ("v", 1), "vz": ("v", 2), "p": ("p", 0), "rho": ("rho", 0)}
+
+
+def probe(params, state, points, field):
+    """FuncAtPoint of `field` (vx, vy, vz, p, rho) of `state` (r, v, rho, p,
+    itype) at `points` (npoints x dim): (status, values)."""
+    L = lib()
+    op = from_params(params)
+    name, comp = PROBE_FIELDS[field]
+    f = np.ascontiguousarray(state[name], dtype=np.float64)
+    stride = f.shape[1] if f.ndim == 2 else 1
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.zeros(len(pts))
+    st = L.oracle_probe(ctypes.byref(op), ptr(np.ascontiguousarray(state["r"], dtype=np.float64)),
+                        ptr(np.ascontiguousarray(state["itype"], dtype=np.int32), ctypes.c_int32), ptr(f), stride, comp,
+                        ptr(pts), len(pts), ptr(out))
+    return st, out
 
 
 def neighbour_sets(nb: np.ndarray, ntotal: int):

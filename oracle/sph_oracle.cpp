@@ -648,6 +648,43 @@ static void per_pair(const oracle_params* p, const uint32_t* nb, F&& f) {
     for (uint32_t j = 0; j < p->ntotal; ++j) for_pairs<D>(*p, nb, j, [&](uint32_t i) { f(j, i); });
 }
 
+// FuncAtPoint (tools/FuncAtPoint/lib/src/fatp.cpp:121-144 calculate, :14-58
+// findValue): the layer's grid_find and density_sum, then at each point the
+// sum over every particle of the 3^dim cells around the point's cell (the
+// grid holds all particles, dead ones too) with |r_i - x|^2 < cell^2 of
+// f_i W_cubic(|r_i - x|) m / rho_i, in the cells' order.
+template <int D>
+static int32_t probe_t(const oracle_params& P, const double* r, const int32_t* itype, const double* f, uint32_t stride,
+                uint32_t comp, const double* pts, uint32_t np, double* out) {
+    std::vector<uint32_t> grid, start, nbr((size_t)P.max_neighbours * P.ntotal);
+    std::vector<uint64_t> key;
+    const int32_t st = grid_find_t<D>(P, r, itype, nbr.data(), grid, start, key);
+    if (st != 0) return st;
+    std::vector<double> rho(P.ntotal), pres(P.ntotal);
+    density_sum_t<D>(P, r, nbr.data(), rho.data(), pres.data());
+    Kern K{P};
+    const double max_d2 = sq(cell_size(P));
+    const uint64_t tsz = table_size(P);
+    const int ncell = D == 3 ? 27 : 9;
+    for (uint32_t g = 0; g < np; ++g) {
+        const V<D> x = V<D>::load(pts, g);
+        uint64_t cells[27];
+        neighbour_cells(D, cell_of<D>(P, x), cells);
+        double val = 0;
+        for (int k = 0; k < ncell; ++k) {
+            const uint64_t c = cells[k];
+            if (c == kInvalidCell || c + 1 >= tsz) continue;
+            for (uint32_t gi = start[c]; gi < start[c + 1]; ++gi) {
+                const uint32_t i = grid[gi];
+                const double d2 = len2(V<D>::load(r, i) - x);
+                if (d2 < max_d2) val += f[(size_t)i * stride + comp] * K.w(std::sqrt(d2), 1) * P.mass / rho[i];
+            }
+        }
+        out[g] = val;
+    }
+    return 0;
+}
+
 extern "C" {
 
 double oracle_kernel_w(const oracle_params* p, double dist, int32_t skf) { return Kern{*p}.w(dist, skf); }
@@ -707,6 +744,13 @@ void oracle_density_con(const oracle_params* p, const double* r, const double* v
     set_threads(p);
     if (p->dim == 3) density_con_t<3>(*p, r, v, nb, rho, drho, pres);
     else density_con_t<2>(*p, r, v, nb, rho, drho, pres);
+}
+
+int32_t oracle_probe(const oracle_params* p, const double* r, const int32_t* itype, const double* field,
+                     uint32_t stride, uint32_t comp, const double* points, uint32_t npoints, double* out) {
+    set_threads(p);
+    return p->dim == 3 ? probe_t<3>(*p, r, itype, field, stride, comp, points, npoints, out)
+                       : probe_t<2>(*p, r, itype, field, stride, comp, points, npoints, out);
 }
 
 // testExternalForceOMP.cpp:33-55: start at (0, -g) and add each pair term

@@ -126,6 +126,13 @@ void oracle_neighbouring_cells(const oracle_params* p, uint64_t idx, uint64_t* c
 uint64_t oracle_count_cells(const oracle_params* p, int32_t* status);
 
 /* ---- step modules (arrays are AoS like the reference: r[dim*i + k]) ---- */
+/* FuncAtPoint (tools/FuncAtPoint/lib/src/fatp.cpp:14-58, :121-144): the SPH
+ * interpolation sum_i f_i W_cubic(|r_i - x|) m / rho_i at each point, rho the
+ * summation density of r; f = field[i * stride + comp].  Returns grid_find's
+ * status (nonzero: no values). */
+int32_t oracle_probe(const oracle_params* p, const double* r, const int32_t* itype, const double* field,
+                     uint32_t stride, uint32_t comp, const double* points, uint32_t npoints, double* out);
+
 int32_t oracle_grid_find(const oracle_params* p, const double* r, const int32_t* itype,
                          uint32_t* neighbours);
 void oracle_density_sum(const oracle_params* p, const double* r, const uint32_t* neighbours,

@@ -127,4 +127,83 @@ __global__ void __launch_bounds__(kBlock) gauge_kernel(uint32_t n, StateA<R> A, 
     if (threadIdx.x == 0 && m > 0.0) atomicMax(&out_bits[blockIdx.y], (unsigned long long)__double_as_longlong(m));
 }
 
+// ------------------------------------------------------------------ probes
+// SPH interpolation of a field at points, as FuncAtPoint does
+// (tools/FuncAtPoint/lib/src/fatp.cpp:14-58): at point x, the sum over the
+// particles of the 3^dim cells around x's cell with |r_i - x|^2 < cell^2 of
+// f_i W(|r_i - x|) m / rho_i, W the cubic spline whatever the step's kernels
+// (SmoothingKernel.h:20-45), rho_i the summation density of the current
+// positions (Density.h:25-61, recomputed per layer at fatp.cpp:134-139).
+// The reference's grid holds every particle, dead ones (itype 0, rho = m W(0),
+// no neighbours) included; here they are keyed past the last cell, so each
+// point also scans that tail [LB[ncells], n).  One thread per point, FP64;
+// fld is the field in original particle order with `stride` values per
+// particle, read at `comp`.
+template <class R>
+__global__ void probe_kernel(StepParams P, StateB<R> B, const uint32_t* __restrict__ LB,
+                             const double* __restrict__ pts, uint32_t np, const double* __restrict__ fld,
+                             uint32_t stride, uint32_t comp, double* __restrict__ out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= np) return;
+    const double x = pts[2 * g], y = pts[2 * g + 1];
+    uint32_t bad = 0;  // points lie in the geometry box (checked on the host)
+    const int cx = cell_coord(x, P.xmin, P.cell, P.ncx, bad);
+    const int cy = cell_coord(y, P.ymin, P.cell, P.ncy, bad);
+    const KernelConsts<double> K = kernel_consts<double>(P);
+    double val = 0.0;
+    auto add = [&](uint32_t i) {
+        const double2 q = B.pos[i];
+        const double d2 = dist2_exact(__dsub_rn(q.x, x), __dsub_rn(q.y, y));
+        if (d2 < P.cut2)
+            val += fld[(size_t)B.oid[i] * stride + comp] * kernel_w<double>(K, 1, sqrt(d2)) * P.mass /
+                   (double)B.pred[i].z;
+    };
+    const int y0 = cy > 0 ? cy - 1 : 0;
+    const int y1 = cy + 1 < P.ncy ? cy + 1 : P.ncy - 1;
+    for (int dc = -1; dc <= 1; ++dc) {
+        const int col = cx + dc;
+        if (col < 0 || col >= P.ncx) continue;
+        for (uint32_t i = LB[col * P.ncy + y0]; i < LB[col * P.ncy + y1 + 1]; ++i) add(i);
+    }
+    for (uint32_t i = LB[P.ncells]; i < P.n; ++i) add(i);
+    out[g] = val;
+}
+
+// The same in 3D (27 cells, rho_i in pred.w).
+template <class R>
+__global__ void probe3_kernel(StepParams P, StateB3<R> B, const uint32_t* __restrict__ LB,
+                              const double* __restrict__ pts, uint32_t np, const double* __restrict__ fld,
+                              uint32_t stride, uint32_t comp, double* __restrict__ out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= np) return;
+    const double x = pts[3 * g], y = pts[3 * g + 1], z = pts[3 * g + 2];
+    uint32_t bad = 0;
+    const int cx = cell_coord(x, P.xmin, P.cell, P.ncx, bad);
+    const int cy = cell_coord(y, P.ymin, P.cell, P.ncy, bad);
+    const int cz = cell_coord(z, P.zmin, P.cell, P.ncz, bad);
+    const KernelConsts<double> K = kernel_consts<double>(P);
+    double val = 0.0;
+    auto add = [&](uint32_t i) {
+        const double4 q = B.pos[i];
+        const double d2 = dist2_exact3(__dsub_rn(q.x, x), __dsub_rn(q.y, y), __dsub_rn(q.z, z));
+        if (d2 < P.cut2)
+            val += fld[(size_t)B.oid[i] * stride + comp] * kernel_w<double>(K, 1, sqrt(d2)) * P.mass /
+                   (double)B.pred[i].w;
+    };
+    const int z0 = cz > 0 ? cz - 1 : 0;
+    const int z1 = cz + 1 < P.ncz ? cz + 1 : P.ncz - 1;
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int xc = cx + dx;
+        if (xc < 0 || xc >= P.ncx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yc = cy + dy;
+            if (yc < 0 || yc >= P.ncy) continue;
+            const uint32_t col = ((uint32_t)xc * (uint32_t)P.ncy + (uint32_t)yc) * (uint32_t)P.ncz;
+            for (uint32_t i = LB[col + z0]; i < LB[col + z1 + 1]; ++i) add(i);
+        }
+    }
+    for (uint32_t i = LB[P.ncells]; i < P.n; ++i) add(i);
+    out[g] = val;
+}
+
 }  // namespace rrsph

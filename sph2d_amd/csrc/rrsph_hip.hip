@@ -896,10 +896,10 @@ rrsph_status rrsph_hip_get_time(rrsph_ctx* ctx, double* time, double* dt, uint64
     return RRSPH_OK;
 }
 
-rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, double* v, double* rho, double* p,
-                                int32_t* itype) {
-    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
-    const size_t n = ctx->P.n, dim = ctx->P.dim;
+// The current state in original particle order into the device staging
+// arrays st_r, st_v, st_rho, st_p, st_itype (what rrsph_hip_download copies).
+static rrsph_status enqueue_unpack(rrsph_ctx* ctx) {
+    const size_t dim = ctx->P.dim;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (dim == 3) {
         if (ctx->fp32)
@@ -920,6 +920,14 @@ rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, doub
             ctx->st_rho, ctx->st_p, ctx->st_itype);
     }
     HIP_TRY(hipGetLastError());
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_download(rrsph_ctx* ctx, uint32_t fields, double* r, double* v, double* rho, double* p,
+                                int32_t* itype) {
+    if (!ctx || !ctx->uploaded) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no state uploaded");
+    const size_t n = ctx->P.n, dim = ctx->P.dim;
+    if (const rrsph_status st = enqueue_unpack(ctx); st != RRSPH_OK) return st;
     if ((fields & RRSPH_FIELD_R) && r)
         HIP_TRY(hipMemcpyAsync(r, ctx->st_r, dim * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     if ((fields & RRSPH_FIELD_V) && v)
@@ -1088,6 +1096,64 @@ rrsph_status rrsph_hip_gauges(rrsph_ctx* ctx, const double* x_gauges, uint32_t n
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, dx + ngauges, ngauges * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_probe(rrsph_ctx* ctx, const double* points, uint32_t npoints, uint32_t field,
+                             double* out) {
+    if (!ctx || !ctx->uploaded || (npoints && (!points || !out)))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    const StepParams& P = ctx->P;
+    const uint32_t dim = (uint32_t)P.dim;
+    // fatp.cpp:92-119 (check_input): a field the layer holds, a point in the geometry box
+    if (field > RRSPH_PROBE_RHO || (field == RRSPH_PROBE_VZ && dim != 3))
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "wrong target value");
+    if (npoints == 0) return RRSPH_OK;
+    if (npoints > RRSPH_MAX_PROBES) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: too many probe points");
+    const double lo[3] = {P.xmin, P.ymin, P.zmin}, hi[3] = {P.xmax, P.ymax, P.zmax};
+    for (uint32_t g = 0; g < npoints; ++g)
+        for (uint32_t d = 0; d < dim; ++d) {
+            const double x = points[(size_t)g * dim + d];
+            if (!(lo[d] <= x && lo[d] + (hi[d] - lo[d]) >= x))
+                return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "point is outside bounds");
+        }
+    // the field as a dump layer holds it
+    if (const rrsph_status st = enqueue_unpack(ctx); st != RRSPH_OK) return st;
+    const double* fld = field == RRSPH_PROBE_P ? ctx->st_p : field == RRSPH_PROBE_RHO ? ctx->st_rho : ctx->st_v;
+    const uint32_t stride = field <= RRSPH_PROBE_VZ ? dim : 1u, comp = field <= RRSPH_PROBE_VZ ? field : 0u;
+    // summation density of the current positions on the sorted copy (never halted)
+    Ctrl zero{};
+    HIP_TRY(hipMemcpyAsync(ctx->util_ctrl, &zero, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    double* dpts = ctx->red;  // dim * npoints points, then npoints results
+    double* dout = dpts + (size_t)dim * npoints;
+    HIP_TRY(hipMemcpyAsync(dpts, points, (size_t)dim * npoints * sizeof(double), hipMemcpyHostToDevice,
+                           ctx->stream));
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd, pb = grid_blocks(npoints, kWave);
+    auto run = [&](auto tag) {
+        using R = decltype(tag);
+        enqueue_util_sort<R>(ctx);
+        if (dim == 3) {
+            StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
+            density_sum3_kernel<R><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, ctx->util_ctrl, B, ctx->sorted_keys,
+                                                                       ctx->LB);
+            probe3_kernel<R><<<pb, kWave, 0, ctx->stream>>>(P, B, ctx->LB, dpts, npoints, fld, stride, comp, dout);
+        } else {
+            StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
+            density_sum_kernel<R><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, ctx->util_ctrl, B, ctx->sorted_keys,
+                                                                      ctx->LB);
+            probe_kernel<R><<<pb, kWave, 0, ctx->stream>>>(P, B, ctx->LB, dpts, npoints, fld, stride, comp, dout);
+        }
+    };
+    if (ctx->fp32) run(float{});
+    else run(double{});
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout, npoints * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    Ctrl u{};
+    HIP_TRY(hipMemcpyAsync(&u, ctx->util_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // the layer's find_neighbours (fatp.cpp:136-137) throws on overflow with the check on
+    if (u.nbr_overflow && P.consistency_check)
+        return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
     return RRSPH_OK;
 }
 

@@ -39,6 +39,7 @@ EXPORTED = [
     "rrsph_hip_stream", "rrsph_hip_set_graph", "rrsph_hip_profile_steps", "rrsph_hip_kernel_class_count",
     "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats",
     "rrsph_hip_smooth_gauges", "rrsph_hip_set_piston", "rrsph_hip_adjoint_forward", "rrsph_hip_adjoint_backward",
+    "rrsph_hip_probe",
     "RRSPH_HIP_GetSpecificVersionMajor",
     "RRSPH_HIP_GetSpecificVersionMinor", "RRSPH_HIP_GetSpecificVersionPatch", "RRSPH_HIP_GetSpecificVersionName",
 ]
@@ -155,6 +156,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "rrsph_hip_neighbours": (st, [vp, up, ctypes.c_uint32]),
         "rrsph_hip_diagnostics": (st, [vp, ctypes.POINTER(Diagnostics)]),
         "rrsph_hip_gauges": (st, [vp, dp, ctypes.c_uint32, ctypes.c_double, ctypes.c_int32, dp]),
+        "rrsph_hip_probe": (st, [vp, dp, ctypes.c_uint32, ctypes.c_uint32, dp]),
         "rrsph_hip_stream": (vp, [vp]),
         "rrsph_hip_set_graph": (st, [vp, ctypes.c_int32]),
         "rrsph_hip_profile_steps": (st, [vp, ctypes.c_uint32, dp, ctypes.c_uint32, dp]),
@@ -305,6 +307,17 @@ class HipStep:
         x = np.ascontiguousarray(xs, dtype=np.float64)
         out = np.zeros(len(x))
         self._check(self.L.rrsph_hip_gauges(self.ctx, _dp(x), len(x), float(search_n), int(ptype), _dp(out)))
+        return out
+
+    PROBE_FIELDS = {"vx": 0, "vy": 1, "vz": 2, "p": 3, "rho": 4}
+
+    def probe(self, points, field: str) -> np.ndarray:
+        """FuncAtPoint (tools/FuncAtPoint/lib/src/fatp.cpp:14-58) of `field`
+        (vx, vy, vz, p, rho) at `points` (npoints x dim) on the current state."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        out = np.zeros(len(pts))
+        code = self.PROBE_FIELDS.get(field, 99)
+        self._check(self.L.rrsph_hip_probe(self.ctx, _dp(pts), len(pts), code, _dp(out)))
         return out
 
     def time_steps(self, n: int) -> float:
