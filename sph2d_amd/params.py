@@ -504,3 +504,99 @@ def start_time_from_dump_name(path: str) -> float:
 
 def params_to_json(params: ExperimentParams) -> Dict[str, Any]:
     return dataclasses.asdict(params)
+
+
+# ---------------------------------------------------------------- writers
+# The parameter files the reference writes into the experiment directory at
+# the start of a run (Input.cpp:323-325).  nlohmann::json keeps object keys
+# sorted and dump(4) indents by four spaces with ": " after each key, which
+# json.dumps(indent=4, sort_keys=True) reproduces; both print a double as its
+# shortest round-trip decimal.  A file ends with a newline (std::endl).
+RRSPH_VERSION = (3, 1, 4)         # CMakeLists.txt:3
+RRSPH_COMMON_VERSION = (1, 3, 4)  # RRSPH_COMMON/CMakeLists.txt:2
+
+# ParticleParams.cpp, params_make_particles_json: every field but depth is
+# printed, the ones a layout did not set as the struct's zero
+# (ParticleParams.h:13-60)
+_PARTICLE_PRINT = ("dim", "x_maxgeom", "x_mingeom", "y_maxgeom", "y_mingeom", "z_mingeom", "z_maxgeom", "delta",
+                   "ntotal", "nfluid", "nvirt", "x_fluid_particles", "y_fluid_particles", "z_fluid_particles",
+                   "x_fluid_min", "x_fluid_max", "y_fluid_min", "y_fluid_max", "z_fluid_min", "z_fluid_max",
+                   "x_boundary_left", "x_boundary_right", "x_boundary_center", "y_boundary_bottom",
+                   "y_boundary_top", "z_boundary_near", "z_boundary_far", "rho0", "nwm_particles_start",
+                   "nwm_particles_end")
+_PARTICLE_INTS = ("dim", "ntotal", "nfluid", "nvirt", "x_fluid_particles", "y_fluid_particles", "z_fluid_particles",
+                  "nwm_particles_start", "nwm_particles_end")
+
+
+def _versions(d: Dict[str, Any]) -> Dict[str, int]:
+    v = PARAMS_VERSION
+    return {"params_target_version_major": int(d.get("params_target_version_major", v[0])),
+            "params_target_version_minor": int(d.get("params_target_version_minor", v[1])),
+            "params_target_version_patch": int(d.get("params_target_version_patch", v[2]))}
+
+
+def model_params_json(mp: Dict[str, Any], versions: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
+    """ModelParams.cpp:313-399 (params_make_model_json): the parsed model
+    parameters, optional ones only when set; this build's Fourier piston
+    keys when set."""
+    out = _versions(versions or {})
+    for name, kind, _default in _MODEL_KEYS:
+        value = mp.get(name)
+        if kind == "optional" and value is None:
+            continue
+        out[name] = value
+    for name in _EXTENSION_KEYS:
+        if mp.get(name) is not None:
+            out[name] = list(mp[name]) if isinstance(mp[name], (list, tuple)) else mp[name]
+    return out
+
+
+def particle_params_json(pp: Dict[str, Any], versions: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
+    """ParticleParams.cpp (params_make_particles_json)."""
+    out = _versions(versions or {})
+    for name in _PARTICLE_PRINT:
+        value = pp.get(name)
+        out[name] = (0 if name in _PARTICLE_INTS else 0.0) if value is None else value
+    if pp.get("depth") is not None:
+        out["depth"] = pp["depth"]
+    return out
+
+
+def computing_params_json(params: ExperimentParams, specific_version=(0, 0, 0, "RRSPH_HIP")) -> Dict[str, Any]:
+    """Input.cpp:140-175 (make_ComputingParams) + ComputingParams.cpp:108-157
+    (params_make_computing_json): derived constants, the wave numbers of the
+    first-order piston when set, and the version stamps."""
+    out: Dict[str, Any] = {
+        "start_simulation_time": params.start_simulation_time, "pi": params.pi, "g": params.g,
+        "TYPE_BOUNDARY": TYPE_BOUNDARY, "TYPE_NON_EXISTENT": TYPE_NON_EXISTENT, "TYPE_WATER": TYPE_WATER,
+        "cell_scale_k": params.cell_scale_k, "max_cells": params.max_cells, "maxn": params.maxn,
+        "hsml": params.hsml, "mass": params.mass,
+    }
+    for name in ("nwm_wave_number", "nwm_freq", "nwm_piston_magnitude"):
+        if getattr(params, name) != 0:
+            out[name] = getattr(params, name)
+    for key, v in (("params_version", PARAMS_VERSION), ("RRSPH_common_version", RRSPH_COMMON_VERSION),
+                   ("RRSPH_version", RRSPH_VERSION), ("RRSPH_specific_version", specific_version)):
+        out[key + "_major"], out[key + "_minor"], out[key + "_patch"] = int(v[0]), int(v[1]), int(v[2])
+    out["RRSPH_specific_version_name"] = str(specific_version[3])
+    return out
+
+
+def dump_params_json(obj: Dict[str, Any]) -> str:
+    """The text nlohmann's json.dump(4) followed by std::endl produces."""
+    return json.dumps(obj, indent=4, sort_keys=True) + "\n"
+
+
+def write_experiment_params(experiment_directory: str, mp: Dict[str, Any], pp: Dict[str, Any],
+                            params: ExperimentParams, specific_version=(0, 0, 0, "RRSPH_HIP"),
+                            versions: Optional[Dict[str, Any]] = None) -> None:
+    """Input.cpp:323-325: ComputingParams.json, ModelParams.json and
+    ParticleParams.json of a run, loadable by load_model_params /
+    load_particle_params here and by the reference's loaders."""
+    os.makedirs(experiment_directory, exist_ok=True)
+    files = (("ComputingParams.json", computing_params_json(params, specific_version)),
+             ("ModelParams.json", model_params_json(mp, versions)),
+             ("ParticleParams.json", particle_params_json(pp, versions)))
+    for name, obj in files:
+        with open(os.path.join(experiment_directory, name), "w") as f:
+            f.write(dump_params_json(obj))

@@ -1,9 +1,12 @@
 """The particle dump CSV (Input.cpp:203-236 reading, Output.cpp:81-167
 writing) and the experiment parameter rules, on the CPU."""
+import json
+import os
+
 import numpy as np
 import pytest
 
-from helpers import load_dambreak_case
+from helpers import GOLDEN, load_dambreak_case
 from sph2d_amd import io as IO
 from sph2d_amd import params as P
 
@@ -64,3 +67,63 @@ def test_golden_case_derived_constants():
     assert prm.cell_scale_k == 2.0
     if prm.eos_sound_vel_method == P.EOS_SOUND_VEL_DAM_BREAK:
         assert prm.eos_c_sqr == pytest.approx(200 * prm.g * prm.depth * prm.eos_sound_vel_coef, rel=1e-15)
+
+
+def _golden_json():
+    d = np.load(os.path.join(GOLDEN, "case_dambreak.npz"))
+    return json.loads(bytes(d["model_params_json"]).decode()), json.loads(bytes(d["particle_params_json"]).decode())
+
+
+def test_model_and_particle_params_round_trip_through_the_writers():
+    """ModelParams.cpp:313-399 / ParticleParams.cpp (the writers): what they
+    print the loaders read back to the same parameters, and every key of the
+    reference's own files comes out with its value."""
+    mj, pj = _golden_json()
+    mp, pp = P.parse_model_params(mj), P.parse_particle_params(pj)
+    wm, wp = P.model_params_json(mp, mj), P.particle_params_json(pp, pj)
+    assert P.parse_model_params(json.loads(P.dump_params_json(wm))) == mp
+    back = P.parse_particle_params(json.loads(P.dump_params_json(wp)))
+    assert {k: back[k] for k in pp} == pp
+    assert all(back[k] == 0 for k in set(back) - set(pp))  # struct fields the layout did not set
+    for k, v in mj.items():
+        if k in dict((n, 1) for n, _, _ in P._MODEL_KEYS):
+            assert wm[k] == v, k
+    for k, v in pj.items():
+        assert wp[k] == v, k
+    # the ParticleParams writer prints every field of the struct, unset ones as 0
+    assert wp["x_fluid_particles"] == 0 and wp["z_boundary_far"] == 0.0 and wp["dim"] == 2
+
+
+def test_computing_params_json_holds_what_the_reference_loader_needs():
+    """ComputingParams.cpp:40-104 (load_computing_params) requires these
+    keys; apply_computing_params (:21-38) takes start time, cell_scale_k,
+    max_cells, maxn, hsml, mass from it."""
+    prm, _st, _tg = load_dambreak_case()
+    cj = json.loads(P.dump_params_json(P.computing_params_json(prm, (1, 2, 3, "RRSPH_HIP"))))
+    for k in ("start_simulation_time", "pi", "g", "TYPE_BOUNDARY", "TYPE_NON_EXISTENT", "TYPE_WATER",
+              "cell_scale_k", "max_cells", "maxn", "hsml", "mass"):
+        assert k in cj, k
+    assert (cj["hsml"], cj["mass"], cj["maxn"], cj["max_cells"], cj["cell_scale_k"]) == \
+        (prm.hsml, prm.mass, prm.maxn, prm.max_cells, prm.cell_scale_k)
+    assert cj["TYPE_BOUNDARY"] == -2 and cj["TYPE_NON_EXISTENT"] == 0 and cj["TYPE_WATER"] == 2
+    assert (cj["RRSPH_specific_version_major"], cj["RRSPH_specific_version_patch"]) == (1, 3)
+    assert cj["RRSPH_specific_version_name"] == "RRSPH_HIP"
+    assert "maxtimestep" not in cj  # make_ComputingParams never sets it
+    assert ("nwm_freq" in cj) == (prm.nwm_freq != 0)
+
+
+def test_params_json_text_is_nlohmann_dump4():
+    """Sorted keys, four-space indent, ': ' separators, shortest round-trip
+    numbers, a final newline (json.dump(4) << std::endl)."""
+    text = P.dump_params_json({"b": 0.009999999776482582, "a": 1000.0, "c": True, "d": 3})
+    assert text == '{\n    "a": 1000.0,\n    "b": 0.009999999776482582,\n    "c": true,\n    "d": 3\n}\n'
+
+
+def test_written_experiment_directory_loads_back(tmp_path):
+    mj, pj = _golden_json()
+    mp, pp = P.parse_model_params(mj), P.parse_particle_params(pj)
+    prm = P.make_params(mp, pp)
+    P.write_experiment_params(str(tmp_path), mp, pp, prm, (0, 1, 0, "RRSPH_HIP"), mj)
+    assert sorted(os.listdir(tmp_path)) == ["ComputingParams.json", "ModelParams.json", "ParticleParams.json"]
+    again = P.make_params(P.load_model_params(str(tmp_path)), P.load_particle_params(str(tmp_path)))
+    assert again == prm
