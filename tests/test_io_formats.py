@@ -127,3 +127,50 @@ def test_written_experiment_directory_loads_back(tmp_path):
     assert sorted(os.listdir(tmp_path)) == ["ComputingParams.json", "ModelParams.json", "ParticleParams.json"]
     again = P.make_params(P.load_model_params(str(tmp_path)), P.load_particle_params(str(tmp_path)))
     assert again == prm
+
+
+def _experiment(tmp_path, r=None, treatment=None):
+    mj, pj = _golden_json()
+    if treatment is not None:
+        mj["consistency_treatment"] = treatment
+    prm, st, _ = load_dambreak_case()
+    (tmp_path / "ModelParams.json").write_text(json.dumps(mj))
+    (tmp_path / "ParticleParams.json").write_text(json.dumps(pj))
+    dump = tmp_path / "dump" / "0.25.csv"
+    IO.write_dump(str(dump), st["r"] if r is None else r, st["itype"], st["v"], st["rho"], lossless=True)
+    return prm, st, str(dump)
+
+
+def test_file_input_restarts_from_a_dump(tmp_path):
+    """fileInput: start time from the dump's stem, the dump's state, the
+    derived and post-fill constants of the reference case, the parameter
+    files rewritten and loadable again."""
+    prm_ref, st, dump = _experiment(tmp_path)
+    prm, state = IO.file_input(dump, str(tmp_path), (0, 1, 0, "RRSPH_HIP"))
+    assert prm.start_simulation_time == 0.25
+    assert np.array_equal(state["r"], st["r"]) and np.array_equal(state["itype"], st["itype"])
+    for k in ("hsml", "mass", "maxn", "max_cells", "cell_scale_k", "depth", "eos_c_sqr", "dt"):
+        assert getattr(prm, k) == getattr(prm_ref, k), k
+    cj = json.loads((tmp_path / "ComputingParams.json").read_text())
+    assert cj["start_simulation_time"] == 0.25 and cj["hsml"] == prm.hsml
+    again = P.make_params(P.load_model_params(str(tmp_path)), P.load_particle_params(str(tmp_path)), 0.25)
+    assert again.hsml == prm.hsml and again.depth == prm.depth
+
+
+def test_file_input_consistency_checks(tmp_path):
+    """A particle outside the box, or a non-finite value, stops the load
+    with the reference's message under STOP (ConsistencyCheck.cpp:62-64)."""
+    _prm, st, _dump = _experiment(tmp_path)
+    r = st["r"].copy()
+    r[5, 0] = 1e3
+    _prm, _st, dump = _experiment(tmp_path, r, P.CONSISTENCY_STOP)
+    with pytest.raises(RuntimeError, match="Encounter particle outside boundaries"):
+        IO.file_input(dump, str(tmp_path), write_params=False)
+    _prm, _st, dump = _experiment(tmp_path, r, P.CONSISTENCY_PRINT)  # PRINT: warn, keep the state
+    with pytest.warns(UserWarning, match="outside boundaries"):
+        _p, state = IO.file_input(dump, str(tmp_path), write_params=False)
+    assert state["r"][5, 0] == 1e3
+    r[5, 0] = np.nan
+    _prm, _st, dump = _experiment(tmp_path, r, P.CONSISTENCY_STOP)
+    with pytest.raises(RuntimeError, match="Encounter not finite value"):
+        IO.file_input(dump, str(tmp_path), write_params=False)
