@@ -86,6 +86,10 @@ struct AdjTape {
     std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
     void* seg = nullptr;                 // states of one replayed segment (every blobs)
     bool seg_last = false;               // seg holds the last segment's states (kept by the forward)
+    // the cell sort each step of seg ran with (keys, permutation): the reverse
+    // step reorders its state with them instead of sorting it again
+    uint32_t *seg_keys = nullptr, *seg_perm = nullptr;
+    std::vector<uint8_t> seg_sorted;     // per seg slot: keys / perm hold that step's sort
     AdjStep* rec = nullptr;              // per step (device)
     std::vector<AdjStep> hrec;           // per step (host copy)
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
@@ -138,6 +142,8 @@ struct rrsph_ctx {
     hipGraphExec_t gexec_full = nullptr;
     const uint32_t* sorted_keys = nullptr;
     const uint32_t* sorted_perm = nullptr;
+    const uint32_t* last_keys = nullptr;  // the cell sort of the last launched step
+    const uint32_t* last_perm = nullptr;
     // incremental re-sort
     uint32_t* skeyA = nullptr;          // key each slot of A was sorted by
     unsigned long long* chg = nullptr;       // changed slots as (key << 32 | slot)
@@ -387,9 +393,16 @@ rrsph_status build_graph(rrsph_ctx* ctx) {
 
 // Sort + reorder of the current state into B (with a scratch pred) for the
 // utility queries, using a control block that is never halted.
+template <class R> void enqueue_util_reorder(rrsph_ctx* c);
 template <class R> void enqueue_util_sort(rrsph_ctx* c) {
     Launcher L{c, nullptr};
     enqueue_sort(c, c->util_ctrl, L);
+    enqueue_util_reorder<R>(c);
+}
+
+// Reorder of the current state by sorted_keys / sorted_perm into B (scratch
+// pred) and the cell-start table, for the utility queries.
+template <class R> void enqueue_util_reorder(rrsph_ctx* c) {
     const uint32_t nb = grid_blocks(c->P.n, kBlock);
     if (c->P.dim == 3) {
         reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm,
@@ -591,7 +604,7 @@ void free_tape(AdjTape& T) {
         if (p) (void)hipFree(p);
     T.ck.clear();
     void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
-                    T.drho_adj, T.out_adj, T.flag, T.rec};
+                    T.drho_adj, T.out_adj, T.flag, T.rec, T.seg_keys, T.seg_perm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     T = AdjTape{};
@@ -619,12 +632,18 @@ void free_all(rrsph_ctx* c) {
 // arithmetic FP64): whole_step reverse, then the pair gather.
 template <class R>
 static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double* lam_in, double* lam_out,
-                                 double* psum) {
+                                 double* psum, const uint32_t* keys, const uint32_t* perm) {
     AdjTape& T = ctx->adj;
     const StepParams& P = ctx->P;
     const size_t n = P.n;
     const uint32_t nb = grid_blocks(P.n, kBlock);
-    enqueue_util_sort<R>(ctx);
+    if (keys) {  // the sort step k ran with, saved by the replay: reorder only
+        ctx->sorted_keys = keys;
+        ctx->sorted_perm = perm;
+        enqueue_util_reorder<R>(ctx);
+    } else {
+        enqueue_util_sort<R>(ctx);
+    }
     if (P.dim == 3) {
         StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
         adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.out_adj + n, T.direct, psum);
@@ -793,6 +812,10 @@ static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
         const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
         if (ctx->use_graph) HIP_TRY(hipGraphLaunch(full ? ctx->gexec_full : ctx->gexec, ctx->stream));
         else enqueue_step(ctx, nullptr, full);
+        // where this step's sort lands: the last radix pass's buffers, or the re-sort's
+        const uint32_t fb = (ctx->passes - 1) & 1u;
+        ctx->last_keys = full ? ctx->kbuf[fb] : ctx->kbuf[0];
+        ctx->last_perm = full ? ctx->vbuf[fb] : ctx->vbuf[0];
         ++(full ? ctx->n_full : ctx->n_incremental);
         ++ctx->launched;
         ctx->need_full = false;
@@ -1313,7 +1336,7 @@ static rrsph_status save_state(rrsph_ctx* ctx, void* dst) {
     return RRSPH_OK;
 }
 
-static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
+static rrsph_status restore_state(rrsph_ctx* ctx, const void* src, bool keys = true) {
     const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
     const char* d = (const char*)src;
     HIP_TRY(hipMemcpyAsync(ctx->posA, d, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
@@ -1323,6 +1346,8 @@ static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + (pb + 2 * vb + 4) * n, sizeof(Ctrl), hipMemcpyDeviceToDevice,
                            ctx->stream));
     // the next step's cell keys from the restored positions, and a full sort
+    ctx->need_full = true;
+    if (!keys) return RRSPH_OK;  // the caller reorders with a saved sort and launches no step
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     if (ctx->P.dim == 3)
         state_keys3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->key_next);
@@ -1333,7 +1358,6 @@ static rrsph_status restore_state(rrsph_ctx* ctx, const void* src) {
         state_keys_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
                                                                   ctx->key_next);
     HIP_TRY(hipGetLastError());
-    ctx->need_full = true;
     return RRSPH_OK;
 }
 
@@ -1373,6 +1397,14 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
         }
         st = launch_steps(ctx, 1);
         if (st != RRSPH_OK) return st;
+        if (!forward || k >= last0) {  // keep the sort of a step whose state seg holds
+            const size_t slot = forward ? k - last0 : k - kslot, n = ctx->P.n;
+            HIP_TRY(hipMemcpyAsync(T.seg_keys + slot * n, ctx->last_keys, n * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(T.seg_perm + slot * n, ctx->last_perm, n * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+            T.seg_sorted[slot] = 1;
+        }
         if (forward) {
             adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec + k);
             st = enqueue_smooth_gauges(ctx, T.spec, T.partials, T.eta + (size_t)k * kMaxGauges,
@@ -1473,6 +1505,9 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
     T.ck.assign(nck + 1, nullptr);
     for (auto& p : T.ck) p = dalloc<char>(T.blob, e);
     T.seg = dalloc<char>(T.blob * T.every, e);
+    T.seg_keys = dalloc<uint32_t>(n * T.every, e);
+    T.seg_perm = dalloc<uint32_t>(n * T.every, e);
+    T.seg_sorted.assign(T.every, 0);
     T.eta = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.zs = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.ws = dalloc<double>((size_t)nsteps * kMaxGauges, e);
@@ -1563,6 +1598,7 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         // replay the segment from its checkpoint, keeping the state before each
         // step (the last segment's states are there already after a forward)
         if (!(T.seg_last && seg == (int)nck - 1)) {
+            std::fill(T.seg_sorted.begin(), T.seg_sorted.end(), (uint8_t)0);
             st = restore_state(ctx, T.ck[seg]);
             if (st != RRSPH_OK) return st;
             st = adjoint_run_checked(ctx, n0, n1, false, T.seg);
@@ -1570,16 +1606,20 @@ rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, d
         }
         T.seg_last = false;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
-            st = restore_state(ctx, (char*)T.seg + (size_t)(k - n0) * T.blob);
+            const size_t slot = (size_t)(k - n0);
+            const bool sorted = T.seg_sorted[slot] != 0;  // the replay launched step k
+            st = restore_state(ctx, (char*)T.seg + slot * T.blob, !sorted);
             if (st != RRSPH_OK) return st;
             // sort and reorder state k exactly as its step did (predict with its dt)
             adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
+            const uint32_t* sk = sorted ? T.seg_keys + slot * n : nullptr;
+            const uint32_t* sp = sorted ? T.seg_perm + slot * n : nullptr;
             if (ctx->fp32)
                 enqueue_reverse_step<float>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                            T.piston_sum + (size_t)k * kMaxSegments);
+                                            T.piston_sum + (size_t)k * kMaxSegments, sk, sp);
             else
                 enqueue_reverse_step<double>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                             T.piston_sum + (size_t)k * kMaxSegments);
+                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp);
             if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur]);  // loss term of state k
             HIP_TRY(hipGetLastError());
             cur = 1 - cur;
