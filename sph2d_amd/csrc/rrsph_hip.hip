@@ -1324,27 +1324,67 @@ static size_t state_bytes(const rrsph_ctx* ctx) {
     return (pos_bytes(ctx) + 2 * real4_bytes(ctx) + 4) * ctx->P.n + sizeof(Ctrl);
 }
 
-static rrsph_status save_state(rrsph_ctx* ctx, void* dst) {
-    const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
-    char* d = (char*)dst;
-    HIP_TRY(hipMemcpyAsync(d, ctx->posA, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + pb * n, ctx->vaA, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + vb) * n, ctx->adA, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + 2 * vb) * n, ctx->oidA, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(d + (pb + 2 * vb + 4) * n, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToDevice,
-                           ctx->stream));
+// Up to kCopyRanges device-to-device copies in one launch (the saved-state
+// blobs of the adjoint move five arrays at a time; separate copies cost a
+// launch each, which dominates at tens of thousands of particles).  Lengths
+// are in 4-byte words; a range moves 16 bytes per access when both ends are
+// 16-byte aligned.
+constexpr int kCopyRanges = 8;
+struct CopyRange {
+    const uint32_t* src;
+    uint32_t* dst;
+    size_t words;
+};
+struct CopyPlan {
+    CopyRange r[kCopyRanges];
+    int n = 0;
+    void add(const void* src, void* dst, size_t bytes) { r[n++] = {(const uint32_t*)src, (uint32_t*)dst, bytes / 4}; }
+};
+__global__ void __launch_bounds__(kBlock) copy_ranges_kernel(CopyPlan plan) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < plan.n; ++k) {
+        const CopyRange c = plan.r[k];
+        size_t done = 0;
+        if (((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst)) & 15u) == 0) {
+            const size_t n4 = c.words / 4;
+            const uint4* __restrict__ s = (const uint4*)c.src;
+            uint4* __restrict__ d = (uint4*)c.dst;
+            for (size_t i = t; i < n4; i += stride) d[i] = s[i];
+            done = n4 * 4;
+        }
+        for (size_t i = done + t; i < c.words; i += stride) c.dst[i] = c.src[i];
+    }
+}
+static rrsph_status enqueue_copies(rrsph_ctx* ctx, const CopyPlan& plan) {
+    size_t units = 0;
+    for (int k = 0; k < plan.n; ++k) units = std::max(units, plan.r[k].words / 4 + 1);
+    const uint32_t blocks = (uint32_t)std::min<size_t>(2048, (units + kBlock - 1) / kBlock);
+    copy_ranges_kernel<<<std::max(blocks, 1u), kBlock, 0, ctx->stream>>>(plan);
+    HIP_TRY(hipGetLastError());
     return RRSPH_OK;
 }
 
-static rrsph_status restore_state(rrsph_ctx* ctx, const void* src, bool keys = true) {
+// the blob of one saved state: posA | vaA | adA | oidA | ctrl
+static CopyPlan state_plan(rrsph_ctx* ctx, void* blob, bool save) {
     const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
-    const char* d = (const char*)src;
-    HIP_TRY(hipMemcpyAsync(ctx->posA, d, pb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->vaA, d + pb * n, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->adA, d + (pb + vb) * n, vb * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->oidA, d + (pb + 2 * vb) * n, 4 * n, hipMemcpyDeviceToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->ctrl, d + (pb + 2 * vb + 4) * n, sizeof(Ctrl), hipMemcpyDeviceToDevice,
-                           ctx->stream));
+    char* d = (char*)blob;
+    void* arrays[5] = {ctx->posA, ctx->vaA, ctx->adA, ctx->oidA, ctx->ctrl};
+    const size_t bytes[5] = {pb * n, vb * n, vb * n, 4 * n, sizeof(Ctrl)};
+    CopyPlan plan;
+    size_t off = 0;
+    for (int k = 0; k < 5; ++k) {
+        if (save) plan.add(arrays[k], d + off, bytes[k]);
+        else plan.add(d + off, arrays[k], bytes[k]);
+        off += bytes[k];
+    }
+    return plan;
+}
+
+static rrsph_status save_state(rrsph_ctx* ctx, void* dst) { return enqueue_copies(ctx, state_plan(ctx, dst, true)); }
+
+static rrsph_status restore_state(rrsph_ctx* ctx, const void* src, bool keys = true) {
+    if (const rrsph_status st = enqueue_copies(ctx, state_plan(ctx, const_cast<void*>(src), false)); st != RRSPH_OK)
+        return st;
     // the next step's cell keys from the restored positions, and a full sort
     ctx->need_full = true;
     if (!keys) return RRSPH_OK;  // the caller reorders with a saved sort and launches no step
@@ -1399,10 +1439,11 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
         if (st != RRSPH_OK) return st;
         if (!forward || k >= last0) {  // keep the sort of a step whose state seg holds
             const size_t slot = forward ? k - last0 : k - kslot, n = ctx->P.n;
-            HIP_TRY(hipMemcpyAsync(T.seg_keys + slot * n, ctx->last_keys, n * sizeof(uint32_t),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(T.seg_perm + slot * n, ctx->last_perm, n * sizeof(uint32_t),
-                                   hipMemcpyDeviceToDevice, ctx->stream));
+            CopyPlan plan;
+            plan.add(ctx->last_keys, T.seg_keys + slot * n, n * sizeof(uint32_t));
+            plan.add(ctx->last_perm, T.seg_perm + slot * n, n * sizeof(uint32_t));
+            st = enqueue_copies(ctx, plan);
+            if (st != RRSPH_OK) return st;
             T.seg_sorted[slot] = 1;
         }
         if (forward) {
