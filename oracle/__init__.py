@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 CPU oracle (oracle/sph_oracle.cpp).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The product package
+cpu_baseline leg of bench.py, as the reference results.  The product package
 (sph2d_amd) never imports this module.
 """
 from __future__ import annotations

@@ -3,7 +3,7 @@
  *
  * This library restates, in plain C++/OpenMP, the RRSPH_OMP time step of
  * RackotRR/SPH2D (the .cpp files of sources/RRSPH_OMP/src and the per-pair
- * physics headers of sources/RRSPH_COMMON/include).  It is the checker that the HIP
+ * physics headers of sources/RRSPH_COMMON/include).  It is what the HIP
  * product path is compared against, and the CPU baseline that bench.py
  * times.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it; the product library never links or calls it.

@@ -114,7 +114,7 @@ _MODEL_KEYS = [
     ("save_density", "default", True),
 ]
 
-# Keys of this build's extension (Fourier-series piston, optionally split
+# Keys of the RRSPH_HIP extension (Fourier-series piston, optionally split
 # into paddles across z); they are ignored by the reference loader and
 # optional here.
 _EXTENSION_KEYS = ("piston_omega", "piston_a", "piston_b", "piston_segments", "piston_seg_zmin", "piston_seg_zmax")
@@ -537,7 +537,7 @@ def _versions(d: Dict[str, Any]) -> Dict[str, int]:
 
 def model_params_json(mp: Dict[str, Any], versions: Optional[Dict[str, Any]] = None) -> Dict[str, Any]:
     """ModelParams.cpp:313-399 (params_make_model_json): the parsed model
-    parameters, optional ones only when set; this build's Fourier piston
+    parameters, optional ones only when set; the RRSPH_HIP Fourier piston
     keys when set."""
     out = _versions(versions or {})
     for name, kind, _default in _MODEL_KEYS:
