@@ -1,0 +1,601 @@
+"""Pins the oracle's small functions with the reference's own known answers
+and with closed forms the mathematics fixes.
+
+* Cell coordinates, Morton keys and neighbouring cells: the tables of
+  sources/RRSPH_COMMON/tests/testGridUtils.cpp:21-257.
+* Smoothing kernels: testSmoothingKernel.cpp:17-192 (h = 1.2, 2D), i.e. the
+  four kernel factors, W and grad W at the reference's points.
+* Tait EOS and its inverse: testEOS.cpp:11-60 (rho0 = 1000, dam-break sound
+  speed with coefficient 2 at depth 50).
+* countCells (Input.cpp:45-75) against an independent bit-by-bit interleave
+  and int.bit_length.
+* delta-SPH (Density.h:102-108): zero for a uniform density, the closed
+  form of the two-particle diffusion term otherwise.
+* Wave makers (WaveMaker.cpp:34-134, cl/TimeIntegration.cl:196-292): the
+  piston velocity is the time derivative of the textbook stroke (first
+  order: Dean & Dalrymple's piston transfer function; second order: Madsen
+  1971; solitary: Rayleigh's paddle displacement as used by Dominguez 2019),
+  and the wall-disappear maker kills exactly its particle range, once.
+
+Every table value is checked with the reference test's own inputs and its
+checkDouble<N> (testSPH.h:51-97), and again at the precision it was printed
+with (the kernel section says how the tables were produced).  A copy of the
+oracle with a planted mistake (a Morton mask bit, a kernel factor, the
+second-order piston coefficient as a sum, the solitary-wave start time, the
+3D cell limit, the edge-of-grid rule, the EOS exponent, the Desbrun support,
+the delta-SPH sign) fails this file.
+"""
+import copy
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import check_double
+from sph2d_amd import generators as G
+from sph2d_amd import params as P
+
+F32_ULP = 2.0 ** -23
+
+
+def _grid_params(hsml=0.5, k=2.0, xmin=0.0, ymin=0.0, dim=2):
+    """testGridUtils.cpp:3-13: hsml 0.5f, cell_scale_k 2, mingeom 0."""
+    op = oracle.OracleParams()
+    op.dim = dim
+    op.hsml = hsml
+    op.cell_scale_k = k
+    op.x_mingeom, op.y_mingeom, op.z_mingeom = xmin, ymin, 0.0
+    op.pi = P.PI
+    return op
+
+
+def _cell_idx(op, x, y):
+    r = np.array([x, y], dtype=np.float64)
+    return int(oracle.lib().oracle_cell_idx(ctypes.byref(op), oracle.ptr(r)))
+
+
+def _neighbour_cells(op, idx):
+    out = (ctypes.c_uint64 * 9)()
+    oracle.lib().oracle_neighbouring_cells(ctypes.byref(op), idx, out)
+    return list(out)
+
+
+INVALID = 0xFFFFFFFF  # GRID_INVALID_CELL = UINT_MAX (GridUtils.h:8)
+
+
+# ---------------------------------------------------------------- grid utils
+def test_cell_coord_from_particle_coord():
+    """testGridUtils.cpp:26-61."""
+    L = oracle.lib()
+    op = _grid_params()
+    cell = 0.5 * 2
+    for r, want in [(0.0, 0), (0.5, 0), (1.5, 1), (2.25, 2), (cell * 1000, 1000), (cell * (1 << 15), 1 << 15)]:
+        assert L.oracle_cell_coord(ctypes.byref(op), r, op.x_mingeom) == want
+        assert L.oracle_cell_coord(ctypes.byref(op), r, op.y_mingeom) == want
+    # mingeom of the other axis does not leak in
+    op = _grid_params(xmin=0.0, ymin=1.0)
+    assert L.oracle_cell_coord(ctypes.byref(op), 0.95, op.x_mingeom) == 0
+    op = _grid_params(xmin=1.0, ymin=0.0)
+    assert L.oracle_cell_coord(ctypes.byref(op), 0.95, op.y_mingeom) == 0
+
+
+MORTON2_TABLE = [  # testGridUtils.cpp:64-93: (x, y) -> key
+    ((41, 18467), 545262667), ((19169, 15724), 451312801), ((11478, 29358), 777574844),
+    ((5705, 28145), 699841091), ((9961, 491), 68615371),
+    ((0, 0), 0), ((1, 0), 1), ((0, 1), 2), ((2, 2), 12),
+]
+
+
+@pytest.mark.parametrize("xy,key", MORTON2_TABLE)
+def test_morton2_encode_decode_table(xy, key):
+    L = oracle.lib()
+    assert L.oracle_morton2(*xy) == key
+    x, y = ctypes.c_uint32(), ctypes.c_uint32()
+    L.oracle_morton2_decode(key, ctypes.byref(x), ctypes.byref(y))
+    assert (x.value, y.value) == xy
+
+
+def test_morton2_boundary_values():
+    """testGridUtils.cpp:96-106: coordinates keep their low 16 bits."""
+    L = oracle.lib()
+    assert L.oracle_morton2(1 << 31, 1) == 2
+    assert L.oracle_morton2(1, 1 << 31) == 1
+    assert L.oracle_morton2(1 << 31, 1 << 31) == 0
+    x, y = ctypes.c_uint32(), ctypes.c_uint32()
+    L.oracle_morton2_decode(1 << 31, ctypes.byref(x), ctypes.byref(y))
+    assert (x.value, y.value) == (0, 32768)
+    L.oracle_morton2_decode(1 << 30, ctypes.byref(x), ctypes.byref(y))
+    assert (x.value, y.value) == (32768, 0)
+
+
+def test_morton2_is_the_bit_interleave():
+    """Closed form: bit b of x goes to bit 2b, bit b of y to bit 2b + 1."""
+    L = oracle.lib()
+    rng = np.random.default_rng(3)
+    for x, y in rng.integers(0, 1 << 16, size=(300, 2)):
+        want = 0
+        for b in range(16):
+            want |= ((int(x) >> b) & 1) << (2 * b) | ((int(y) >> b) & 1) << (2 * b + 1)
+        assert L.oracle_morton2(int(x), int(y)) == want
+
+
+def test_morton3_decode_inverts_encode():
+    L = oracle.lib()
+    rng = np.random.default_rng(4)
+    x, y, z = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    for a, b, c in rng.integers(0, 1024, size=(200, 3)):
+        key = L.oracle_morton3(int(a), int(b), int(c))
+        L.oracle_morton3_decode(key, ctypes.byref(x), ctypes.byref(y), ctypes.byref(z))
+        assert (x.value, y.value, z.value) == (a, b, c)
+
+
+CELL_IDX_TABLE = [  # testGridUtils.cpp:115-124 (float literals are exact here)
+    ((41.0, 467.0), 173643), ((334.0, 500.0), 244340), ((169.0, 724.0), 583265),
+    ((26962.0, 24464.0), 921408260), ((2995.0, 11942.0), 149802285), ((0.0, 0.0), 0),
+    ((0.5, 0.25), 0), ((1.5, 0.25), 1), ((0.75, 2.95), 8), ((0.5, 1.4), 2),
+]
+
+
+@pytest.mark.parametrize("r,key", CELL_IDX_TABLE)
+def test_cell_idx_table(r, key):
+    op = _grid_params()
+    # 2.95f and 1.4f as the reference's float literals
+    x, y = (float(np.float32(v)) for v in r)
+    assert _cell_idx(op, x, y) == key
+
+
+NEIGHBOUR_CELLS = [  # testGridUtils.cpp:128-257: (x, y), invalid count, cells that must be present
+    ((10.0, 0.0), 3, [68, 70, 65, 69, 67, 71]),
+    ((0.0, 10.0), 3, [136, 138, 130, 137, 139, 131]),
+    ((10.0, 65535.5), 3, [2863311598, 2863311596, 2863311599, 2863311595, 2863311597, 2863311593]),
+    ((65535.5, 20.0), 3, [1431656309, 1431656311, 1431656287, 1431656308, 1431656310, 1431656286]),
+    ((0.0, 0.0), 5, [0, 1, 2, 3]),
+    ((65535.5, 0.0), 5, [1431655765, 1431655767, 1431655764, 1431655766]),
+    ((0.0, 65535.5), 5, [2863311530, 2863311528, 2863311531, 2863311529]),
+    ((65534.5, 65535.5), 4, [4294967289, 4294967291, 4294967292, 4294967293, 4294967294]),
+    ((65535.5, 65534.5), 4, [4294967286, 4294967287, 4294967292, 4294967293, 4294967294]),
+    ((65535.5, 65535.5), 9, []),
+]
+
+
+@pytest.mark.parametrize("r,ninvalid,present", NEIGHBOUR_CELLS)
+def test_neighbouring_cells_at_the_grid_edges(r, ninvalid, present):
+    op = _grid_params()
+    cells = _neighbour_cells(op, _cell_idx(op, *r))
+    assert cells.count(INVALID) == ninvalid
+    for c in present:
+        assert c in cells
+
+
+def test_neighbouring_cells_are_x_major():
+    """GridUtils.h:209-214: x outer, y inner, each from -1 to +1."""
+    L = oracle.lib()
+    op = _grid_params()
+    cells = _neighbour_cells(op, L.oracle_morton2(100, 200))
+    want = [L.oracle_morton2(100 + dx, 200 + dy) for dx in (-1, 0, 1) for dy in (-1, 0, 1)]
+    assert cells == want
+
+
+# ---------------------------------------------------------------- countCells
+def _interleave2(x, y):
+    key = 0
+    for b in range(16):
+        key |= ((x >> b) & 1) << (2 * b) | ((y >> b) & 1) << (2 * b + 1)
+    return key
+
+
+def _interleave3(x, y, z):
+    key = 0
+    for b in range(10):
+        key |= ((x >> b) & 1) << (3 * b) | ((y >> b) & 1) << (3 * b + 1) | ((z >> b) & 1) << (3 * b + 2)
+    return key
+
+
+@pytest.mark.parametrize("extent", [(10.0, 10.0), (3.2, 1.6), (20.0, 0.9), (1000.5, 3.0), (65535.0, 65536.0),
+                                    (1.0, 77777.0)])
+def test_count_cells_2d(extent):
+    """Input.cpp:45-75: max_cells = 1 << (intlog2(morton(x_id, y_id)) + 1),
+    x_id = floor(extent / cell); more than 2^16 cells per axis is the error."""
+    op = _grid_params(hsml=0.5)
+    op.x_mingeom, op.y_mingeom = -1.0, -2.0
+    op.x_maxgeom, op.y_maxgeom = op.x_mingeom + extent[0], op.y_mingeom + extent[1]
+    st = ctypes.c_int32(-1)
+    got = oracle.lib().oracle_count_cells(ctypes.byref(op), ctypes.byref(st))
+    x_id, y_id = int(extent[0] // 1.0), int(extent[1] // 1.0)
+    if x_id > 65536 or y_id > 65536:
+        assert st.value == 4 and got == 0
+        return
+    assert st.value == 0
+    key = _interleave2(x_id & 0xFFFF, y_id & 0xFFFF)
+    assert got == 1 << max(key.bit_length() - 1, 0) + 1
+
+
+@pytest.mark.parametrize("extent,ok", [((1.6, 0.8, 0.6), True), ((1023.5, 2.0, 3.0), True),
+                                       ((1025.0, 2.0, 3.0), False), ((5.0, 5.0, 1024.0), False),
+                                       ((5.0, 5.0, 1023.9), True)])
+def test_count_cells_3d(extent, ok):
+    """3D: 10 bits per axis; x, y past 1024 or z at 1024 and above is the error."""
+    op = _grid_params(hsml=0.5, dim=3)
+    op.x_maxgeom, op.y_maxgeom, op.z_maxgeom = extent
+    st = ctypes.c_int32(-1)
+    got = oracle.lib().oracle_count_cells(ctypes.byref(op), ctypes.byref(st))
+    if not ok:
+        assert st.value == 4 and got == 0
+        return
+    ids = [int(e // 1.0) for e in extent]
+    key = _interleave3(*(i & 0x3FF for i in ids))
+    assert st.value == 0 and got == 1 << max(key.bit_length() - 1, 0) + 1
+
+
+# ---------------------------------------------------------------- smoothing kernels
+# The reference's tables were printed by SPH2D v3.0.15, which evaluated the
+# kernels with pi rounded to single precision (3.1415927410125732) and, in
+# the cubic W, with single-precision 2/3 and 1/6; the float-literal gradient
+# targets were computed at diff = (1.5, -1.2) h exactly, while the test feeds
+# 1.2f.  The current reference (Params.h:190, SmoothingKernel.h:14-242) uses
+# double pi and double constants, as the oracle does.  So every table value
+# is checked twice: with the reference's own test inputs and checkDouble<6>
+# (what its suite asserts), and tightly with pi = float(pi) and exact inputs,
+# where the oracle must reproduce the table to the precision it was printed
+# with (one float ulp for float literals, 1e-15 for doubles).  Only the cubic
+# W keeps a 1.3e-7 residual there: its single-precision 2/3 and 1/6.
+HSML = 1.2
+PI_F32 = float(np.float32(math.pi))
+
+
+def _skf_params(pi=P.PI):
+    """testSmoothingKernel.cpp:8-15: hsml 1.2, 2D."""
+    op = _grid_params(hsml=HSML)
+    op.pi = pi
+    return op
+
+
+def _w(skf, dist, pi=P.PI):
+    op = _skf_params(pi)
+    return oracle.lib().oracle_kernel_w(ctypes.byref(op), dist, skf)
+
+
+def _dwdr(skf, diff, pi=P.PI):
+    op = _skf_params(pi)
+    d = np.ascontiguousarray(diff, dtype=np.float64)
+    out = np.zeros(2)
+    oracle.lib().oracle_kernel_dwdr(ctypes.byref(op), float(np.hypot(*d)), oracle.ptr(d), skf, oracle.ptr(out))
+    return out
+
+
+def _close(got, want, rel):
+    return abs(got - want) <= rel * abs(want)
+
+
+FACTORS = {P.SKF_CUBIC: 0.47367541268751695, P.SKF_GAUSS: 0.22104852592084123,
+           P.SKF_WENDLAND: 0.38683492036147216, P.SKF_DESBRUN: 0.06907766435026289}  # :29-35
+
+
+def _factors(pi):
+    """Each factor read off W where the kernel's shape is known: cubic W(0) =
+    (2/3) f, Gauss W(0) = f, Wendland W(0) = f, Desbrun W(h) = f."""
+    return {P.SKF_CUBIC: _w(1, 0.0, pi) * 1.5, P.SKF_GAUSS: _w(2, 0.0, pi), P.SKF_WENDLAND: _w(3, 0.0, pi),
+            P.SKF_DESBRUN: _w(4, HSML, pi)}
+
+
+def test_kernel_factors():
+    """testSmoothingKernel.cpp:29-35; closed forms 15/(7 pi h^2), 1/(pi h^2),
+    7/(4 pi h^2), 5/(16 pi h^2)."""
+    got, got_f = _factors(P.PI), _factors(PI_F32)
+    closed = {1: 15 / (7 * math.pi * HSML ** 2), 2: 1 / (math.pi * HSML ** 2), 3: 7 / (4 * math.pi * HSML ** 2),
+              4: 5 / (16 * math.pi * HSML ** 2)}
+    for skf, want in FACTORS.items():
+        assert check_double(got[skf], want, 6), skf
+        assert _close(got[skf], closed[skf], 1e-15), skf
+        if skf != P.SKF_CUBIC:
+            assert _close(got_f[skf], want, 1e-15), skf
+    # the cubic factor itself is exact too; only its W(0) carries the float 2/3
+    assert _close(15 / (7 * PI_F32 * HSML ** 2), FACTORS[P.SKF_CUBIC], 1e-15)
+
+
+W_TABLE = {  # testSmoothingKernel.cpp:38-110, at q = 0, 0.5, ..., 3.5
+    P.SKF_CUBIC: [0.31578361786942954, 0.2269694779905201, 0.07894591152567106, 0.009868238058419673, 0, 0, 0, 0],
+    P.SKF_GAUSS: [0.22104852592084123, 0.1721527650839309, 0.08131920818753016, 0.023298343222599834,
+                  0.004048644977653125, 0.00042672404117092595, 2.7279555277540326e-05, 1.0577431458882936e-06],
+    P.SKF_WENDLAND: [0.38683492036147216, 0.2447939730412441, 0.07253154756777602, 0.0060442956306480024,
+                     0, 0, 0, 0],
+    P.SKF_DESBRUN: [0.5526213148021031, 0.23313711718213723, 0.06907766435026289, 0.00863470804378286, 0, 0, 0, 0],
+}
+
+
+@pytest.mark.parametrize("skf", sorted(W_TABLE))
+def test_kernel_w_values(skf):
+    tight = 1.3e-7 if skf == P.SKF_CUBIC else 1e-15
+    for k, want in enumerate(W_TABLE[skf]):
+        got = _w(skf, 0.5 * k * HSML)
+        assert check_double(got, want, 6 if want else 5), (skf, k)
+        got_f = _w(skf, 0.5 * k * HSML, PI_F32)
+        if want:
+            assert _close(got_f, want, tight), (skf, k, got_f, want)
+        else:
+            assert got == 0.0 and got_f == 0.0
+
+
+def _f32(v):
+    return float(np.float32(v))
+
+
+# testSmoothingKernel.cpp:138-192: diff in units of h, and the targets
+DWDR_POINTS = [(0.5, -1.0), (-0.5, 1.0), (1.5, -1.2), (-1.5, 1.2), (2.5, -1.0), (-2.5, 1.0)]
+DWDR_TABLE = {
+    P.SKF_CUBIC: [(-0.06865754906937419, 0.13731509813874837), (0.06865754906937419, -0.13731509813874837),
+                  (-0.000963365912832955, 0.0007706927302663641), (0.000963365912832955, -0.0007706927302663641),
+                  (0.0, 0.0), (0.0, 0.0)],
+    P.SKF_GAUSS: [(-0.05277621917932923, 0.10555243835865846), (0.05277621917932923, -0.10555243835865846),
+                  (-0.013800060601843458, 0.011040048481474767), (0.013800060601843458, -0.011040048481474767),
+                  (-0.0006540958408349164, 0.00026163833633396655), (0.0006540958408349164, -0.00026163833633396655)],
+    P.SKF_WENDLAND: [(-0.06911144827074141, 0.13822289654148281), (0.06911144827074141, -0.13822289654148281),
+                     (-0.00014935889800846844, 0.00011948711840677473),
+                     (0.00014935889800846844, -0.00011948711840677473), (0.0, 0.0), (0.0, 0.0)],
+    P.SKF_DESBRUN: [(-0.060075355435702416, 0.12015071087140483), (0.060075355435702416, -0.12015071087140483),
+                    (-0.0008429451737288357, 0.0006743561389830686), (0.0008429451737288357, -0.0006743561389830686),
+                    (0.0, 0.0), (0.0, 0.0)],
+}
+
+
+@pytest.mark.parametrize("skf", sorted(DWDR_TABLE))
+def test_kernel_dwdr_values(skf):
+    floats = skf != P.SKF_DESBRUN  # cubic, Gauss, Wendland: float-literal inputs and targets
+    for (cx, cy), want in zip(DWDR_POINTS, DWDR_TABLE[skf]):
+        # the reference test's inputs (1.5f * hsml, -1.2f * hsml) and its own check
+        lit = np.array([(_f32(cx) if floats else cx) * HSML, (_f32(cy) if floats else cy) * HSML])
+        got = _dwdr(skf, lit)
+        tgt = [_f32(w) for w in want] if floats else list(want)
+        for g, w in zip(got, tgt):
+            assert check_double(g, w, 6), (skf, lit, got, want)
+        # exact inputs and single-precision pi: the table to its printed precision
+        got_f = _dwdr(skf, np.array([cx * HSML, cy * HSML]), PI_F32)
+        for g, w in zip(got_f, tgt):
+            tol = (F32_ULP if floats else 1e-15) * abs(w)
+            assert abs(g - w) <= tol, (skf, cx, cy, g, w)
+
+
+def test_kernel_dwdr_at_zero():
+    """testSmoothingKernel.cpp:138-192: zero gradient at diff = 0, except
+    Desbrun, whose diff / |diff| is NaN there."""
+    for skf in (P.SKF_CUBIC, P.SKF_GAUSS, P.SKF_WENDLAND):
+        assert np.array_equal(_dwdr(skf, [0.0, 0.0]), [0.0, 0.0])
+    assert np.isnan(_dwdr(P.SKF_DESBRUN, [0.0, 0.0])).all()
+
+
+@pytest.mark.parametrize("skf", [1, 2, 3, 4])
+def test_kernel_dwdr_is_the_gradient_of_w(skf):
+    """grad W = W'(|r|) r / |r| (central differences of W), and W integrates
+    to one over the plane (2D normalisation)."""
+    rng = np.random.default_rng(skf)
+    for _ in range(20):
+        diff = rng.uniform(-2.2, 2.2, 2) * HSML
+        d = float(np.hypot(*diff))
+        eps = 1e-6 * HSML
+        fd = (_w(skf, d + eps) - _w(skf, d - eps)) / (2 * eps)
+        assert np.allclose(_dwdr(skf, diff), fd * diff / d, rtol=1e-6, atol=1e-9)
+    rr = np.linspace(0.0, 4.0 * HSML, 40001)
+    w = np.array([_w(skf, x) for x in rr])
+    mass = np.trapezoid(2 * np.pi * rr * w, rr)
+    assert abs(mass - 1.0) < (2e-4 if skf == P.SKF_GAUSS else 1e-6)
+
+
+# ---------------------------------------------------------------- EOS
+def _eos_params():
+    """testEOS.cpp:11-17: rho0 1000, dam-break sound speed, coefficient 2, depth 50."""
+    prm = P.ExperimentParams()
+    prm.rho0 = 1000.0
+    prm.eos_sound_vel_method = P.EOS_SOUND_VEL_DAM_BREAK
+    prm.eos_sound_vel_coef = 2.0
+    prm.depth = 50.0
+    op = oracle.OracleParams()
+    op.rho0 = prm.rho0
+    op.eos_c_sqr = prm.eos_c_sqr
+    op.eos_c = prm.eos_c
+    return prm, op
+
+
+def test_eos_sound_speed():
+    """testEOS.cpp:24-31: c = sqrt(200 g d coef) = 442.9447."""
+    prm, _ = _eos_params()
+    assert check_double(prm.eos_c, 442.9447, 3)
+    assert abs(prm.eos_c - 442.9447) < 5e-5
+    assert prm.eos_c_sqr == pytest.approx(prm.eos_c ** 2, rel=1e-15)
+    assert prm.eos_c_sqr == 200 * 9.81 * 50 * 2
+
+
+# testEOS.cpp:33-46: rho -> p, printed with 6 to 8 significant digits
+EOS_P_TABLE = [(500, "-27809600"), (995, "-966407.06"), (999, "-195612.39"), (1001, "196789.59"),
+               (1005, "995838.31"), (1500, "4.5086586e+08"), (2000, "3.5596288e+09")]
+
+
+def _half_unit(literal: str) -> float:
+    """Half a unit of the last significant printed digit of `literal`."""
+    mant, _, exp = literal.lstrip("-").partition("e")
+    e = int(exp) if exp else 0
+    if "." in mant:
+        places = len(mant.split(".")[1])
+    else:
+        places = -(len(mant) - len(mant.rstrip("0")))
+    return 0.5 * 10.0 ** (e - places)
+
+
+def test_eos_p_values():
+    _, op = _eos_params()
+    L = oracle.lib()
+    B = op.eos_c_sqr * op.rho0 / 7
+    assert L.oracle_eos_p(ctypes.byref(op), 0.0) == -B
+    assert L.oracle_eos_p(ctypes.byref(op), 1000.0) == 0.0
+    for rho, lit in EOS_P_TABLE:
+        got = L.oracle_eos_p(ctypes.byref(op), float(rho))
+        assert check_double(got, float(lit), 3)
+        assert abs(got - float(lit)) <= _half_unit(lit) + 1e-7 * abs(float(lit)), (rho, got, lit)
+        assert got == pytest.approx(B * ((rho / 1000.0) ** 7 - 1), rel=1e-14)
+
+
+def test_eos_rho_values():
+    """testEOS.cpp:48-60, and the inverse in closed form: rho0 (1 + p/B)^(1/7)."""
+    _, op = _eos_params()
+    L = oracle.lib()
+    B = op.eos_c_sqr * op.rho0 / 7
+    assert math.isnan(L.oracle_eos_rho(ctypes.byref(op), -2 * B))
+    assert L.oracle_eos_rho(ctypes.byref(op), -B) == 0.0
+    assert L.oracle_eos_rho(ctypes.byref(op), 0.0) == 1000.0
+    for p, rho in [(-27809600, 500), (-966407.06, 995), (196789.59, 1001), (995838.31, 1005), (B, 1104),
+                   (2 * B, 1170)]:
+        assert check_double(L.oracle_eos_rho(ctypes.byref(op), float(p)), rho, 3)
+    assert L.oracle_eos_rho(ctypes.byref(op), B) == pytest.approx(1000 * 2 ** (1 / 7), rel=1e-15)
+    assert L.oracle_eos_rho(ctypes.byref(op), 2 * B) == pytest.approx(1000 * 3 ** (1 / 7), rel=1e-15)
+    for rho in (950.0, 1000.0, 1003.5, 1200.0):
+        assert L.oracle_eos_rho(ctypes.byref(op), L.oracle_eos_p(ctypes.byref(op), rho)) == pytest.approx(
+            rho, rel=1e-12)
+
+
+# ---------------------------------------------------------------- delta-SPH
+def _pair_case(rho_i, rho_j, q=0.8, angle=0.7, v=None):
+    lay = G.dam_break()
+    prm = copy.copy(lay.params)
+    prm.density_treatment = P.DENSITY_CONTINUITY_DELTA
+    prm.density_delta_sph_coef = 0.1
+    prm.ntotal = prm.nfluid = 2
+    c = np.array([1.0, 0.5])
+    d = q * prm.hsml * np.array([math.cos(angle), math.sin(angle)])
+    r = np.array([c, c + d])  # particle 0 = j, particle 1 = i
+    vel = np.zeros((2, 2)) if v is None else np.asarray(v, dtype=np.float64)
+    rho = np.array([rho_j, rho_i])
+    itype = np.array([2, 2], dtype=np.int32)
+    st, nb = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    op = oracle.from_params(prm)
+    drho, p = np.zeros(2), np.zeros(2)
+    nbf = np.ascontiguousarray(nb.reshape(-1))
+    oracle.lib().oracle_density_con(ctypes.byref(op), oracle.ptr(np.ascontiguousarray(r)),
+                                    oracle.ptr(np.ascontiguousarray(vel)), oracle.ptr(nbf, ctypes.c_uint32),
+                                    oracle.ptr(rho), oracle.ptr(drho), oracle.ptr(p))
+    return prm, q * prm.hsml, drho
+
+
+def test_delta_sph_vanishes_for_uniform_density():
+    _prm, _d, drho = _pair_case(1000.0, 1000.0)
+    assert np.array_equal(drho, [0.0, 0.0])
+
+
+@pytest.mark.parametrize("q", [0.3, 0.8, 1.4])
+def test_delta_sph_two_particle_closed_form(q):
+    """Density.h:102-108 for two particles at rest: the continuity term is
+    zero and d rho_j/dt = 2 (coef h c) m (rho_i - rho_j)/rho_i (-W'(r)/r),
+    W' the radial derivative of the cubic spline (f/h (-2q + 3q^2/2) for
+    q <= 1, -f/h (2 - q)^2/2 for 1 < q <= 2).  Density diffuses from the
+    denser particle to the lighter one."""
+    rho_j, rho_i = 1000.0, 1012.0
+    prm, r, drho = _pair_case(rho_i, rho_j, q=q)
+    h, f = prm.hsml, 15.0 / (7.0 * math.pi * prm.hsml ** 2)
+    wp = f / h * (-2 * q + 1.5 * q * q) if q <= 1 else -f / h * 0.5 * (2 - q) ** 2
+    dhc = prm.density_delta_sph_coef * h * prm.eos_c
+    want_j = 2 * dhc * prm.mass * (rho_i - rho_j) / rho_i * (-wp / r)
+    want_i = 2 * dhc * prm.mass * (rho_j - rho_i) / rho_j * (-wp / r)
+    assert want_j > 0 > want_i
+    assert drho[0] == pytest.approx(want_j, rel=1e-12)
+    assert drho[1] == pytest.approx(want_i, rel=1e-12)
+
+
+# ---------------------------------------------------------------- wave makers
+def _piston_params(nwm, H=0.1, L=2.0, depth=0.5, phase=0.3, t0=0.25):
+    prm = P.ExperimentParams()
+    prm.nwm = nwm
+    prm.depth = depth
+    prm.nwm_wave_length = L
+    prm.nwm_wave_magnitude = H
+    prm.nwm_phase = phase
+    prm.nwm_time_start = t0
+    prm.hsml = prm.delta = 0.01
+    P.post_fill_in_model_params(prm)
+    return prm
+
+
+def _velocity(prm, t):
+    op = oracle.from_params(prm)
+    return oracle.lib().oracle_piston_velocity(ctypes.byref(op), t)
+
+
+def _derivative(x, t, eps=1e-5):
+    return (x(t + eps) - x(t - eps)) / (2 * eps)
+
+
+def test_first_order_piston_stroke_derivative():
+    """nwm 2: x(t) = (S0 / 2) sin(w t + phase - w t0), with the piston
+    transfer function H / S0 = 2 (cosh 2kd - 1) / (sinh 2kd + 2kd) (Dean &
+    Dalrymple) and w^2 = g k tanh(kd)."""
+    prm = _piston_params(P.NWM_METHOD_DYNAMIC_1)
+    k, d = 2 * math.pi / prm.nwm_wave_length, prm.depth
+    w = math.sqrt(9.81 * k * math.tanh(k * d))
+    S0 = prm.nwm_wave_magnitude * (math.sinh(2 * k * d) + 2 * k * d) / (2 * (math.cosh(2 * k * d) - 1))
+    x = lambda t: 0.5 * S0 * math.sin(w * t + prm.nwm_phase - w * prm.nwm_time_start)
+    for t in np.linspace(0.3, 3.0, 17):
+        assert _velocity(prm, t) == pytest.approx(_derivative(x, t), rel=1e-8, abs=1e-11)
+
+
+def test_second_order_piston_stroke_derivative():
+    """nwm 3 (Madsen 1971): x(t) = (S0/2) sin(th) + e2 sin(2 th), th = w t +
+    phase - w t0, e2 = H^2/(32 d) (3 cosh kd / sinh^3 kd - 2 / m1), m1 = H/S0
+    (the product form of cl/TimeIntegration.cl:217)."""
+    prm = _piston_params(P.NWM_METHOD_DYNAMIC_2, H=0.12, L=3.0, depth=0.6)
+    k, d, H = 2 * math.pi / prm.nwm_wave_length, prm.depth, prm.nwm_wave_magnitude
+    w = math.sqrt(9.81 * k * math.tanh(k * d))
+    m1 = 2 * (math.cosh(2 * k * d) - 1) / (math.sinh(2 * k * d) + 2 * k * d)
+    e2 = H * H / (32 * d) * (3 * math.cosh(k * d) / math.sinh(k * d) ** 3 - 2 / m1)
+    th = lambda t: w * t + prm.nwm_phase - w * prm.nwm_time_start
+    x = lambda t: 0.5 * H / m1 * math.sin(th(t)) + e2 * math.sin(2 * th(t))
+    first = lambda t: 0.5 * H / m1 * w * math.cos(th(t))
+    seen = 0.0
+    for t in np.linspace(0.3, 3.0, 17):
+        v = _velocity(prm, t)
+        assert v == pytest.approx(_derivative(x, t), rel=1e-8, abs=1e-11)
+        seen = max(seen, abs(v - first(t)))
+    assert seen > 1e-4  # the second-order term is present
+
+
+def test_solitary_paddle_displacement_derivative():
+    """nwm 5: Rayleigh solitary-wave paddle x(t) = H u / (k (h + H - H u^2)),
+    u = tanh(k c (t - Tf)), k = sqrt(3H / (4 h^2 (H + h))), c = sqrt(g (H + h)),
+    Tf = 2/(k c) (3.8 + H/h); the velocity is its time derivative and the
+    paddle travels 2H/(k h) in total."""
+    prm = _piston_params(P.NWM_METHOD_SOLITARY_RAYLEIGH, H=0.08, depth=0.4)
+    H, h = prm.nwm_wave_magnitude, prm.depth
+    k = math.sqrt(3 * H / (4 * h * h * (H + h)))
+    c = math.sqrt(9.81 * (H + h))
+    Tf = 2 / (k * c) * (3.8 + H / h)
+    x = lambda t: H * math.tanh(k * c * (t - Tf)) / (k * (h + H - H * math.tanh(k * c * (t - Tf)) ** 2))
+    for t in np.linspace(0.0, 2 * Tf, 23):
+        assert _velocity(prm, t) == pytest.approx(_derivative(x, t, 1e-6), rel=1e-7, abs=1e-10)
+    stroke = x(50 * Tf) - x(-50 * Tf)
+    assert stroke == pytest.approx(2 * H / (k * h), rel=1e-12)
+    assert _velocity(prm, Tf) == pytest.approx(c * H / (h + H), rel=1e-12)  # u = 0: c eta / (h + eta)
+
+
+def test_wall_disappear_kills_exactly_its_range_once():
+    """nwm 4 with boundary motion (CLTimeIntegration.cpp:397-401): at the
+    first step with time >= nwm_time_start the particles of
+    [nwm_particles_start, nwm_particles_end) become non-existent; no other
+    particle changes type, and nothing dies before that step."""
+    lay = G.dam_break()
+    prm = copy.copy(lay.params)
+    prm.boundary_motion = True
+    prm.nwm = P.NWM_METHOD_WALL_DISAPPEAR
+    prm.nwm_particles_start = prm.nfluid + 10
+    prm.nwm_particles_end = prm.nfluid + 130
+    prm.nwm_time_start = 2.5 * prm.dt
+    ora = oracle.Oracle(prm)
+    ora.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=prm.dt)
+    assert ora.step(3) == 0  # times 0, dt, 2 dt: before the start
+    assert np.array_equal(ora.state()["itype"], lay.itype)
+    assert ora.step(1) == 0
+    it = ora.state()["itype"]
+    dead = np.zeros(prm.ntotal, dtype=bool)
+    dead[prm.nwm_particles_start:prm.nwm_particles_end] = True
+    assert np.all(it[dead] == 0)
+    assert np.array_equal(it[~dead], lay.itype[~dead])
+    assert ora.step(5) == 0
+    assert np.array_equal(ora.state()["itype"], it)
