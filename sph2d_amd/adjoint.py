@@ -63,14 +63,22 @@ class PistonGauges:
             class _Fn(torch.autograd.Function):
                 @staticmethod
                 def forward(ctx, a_t, b_t, run_obj):
-                    eta = run_obj.forward(a_t.detach().double().cpu().numpy(), b_t.detach().double().cpu().numpy())
+                    a_np = a_t.detach().double().cpu().numpy().copy()
+                    b_np = b_t.detach().double().cpu().numpy().copy()
+                    eta = run_obj.forward(a_np, b_np)
                     ctx.run = run_obj
+                    ctx.coefs = (a_np, b_np)
+                    ctx.tape = run_obj.sim.tape_generation
                     ctx.ncoef = a_t.numel()
                     ctx.dtype, ctx.device = a_t.dtype, a_t.device
                     return torch.from_numpy(eta).to(dtype=torch.float64, device=a_t.device)
 
                 @staticmethod
                 def backward(ctx, eta_bar):
+                    # another forward on the same context has replaced this
+                    # call's tape since: record this call's trajectory again
+                    if ctx.run.sim.tape_generation != ctx.tape:
+                        ctx.run.forward(*ctx.coefs)
                     g = ctx.run.backward(eta_bar.detach().double().cpu().numpy(), ctx.ncoef)
                     ga = torch.from_numpy(g["a"]).to(dtype=ctx.dtype, device=ctx.device)
                     gb = torch.from_numpy(g["b"]).to(dtype=ctx.dtype, device=ctx.device)

@@ -63,7 +63,7 @@ struct Ctrl {
     uint32_t cur_first;     // step == 0 -> v_dt = dt / 2
     uint32_t piston_active;
     uint32_t wall_disappear;
-    uint32_t pad1;
+    uint32_t rollback;      // 1: the halted step was integrated by the fused pair pass; the host restores its input
     double piston_vseg[8];  // segmented piston: velocity of each paddle
 };
 

@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restr
                                                          const unsigned long long* __restrict__ mask,
                                                          const uint32_t* __restrict__ wpre, uint32_t n,
                                                          uint32_t* __restrict__ skeyB, uint32_t* __restrict__ perm,
-                                                         const Ctrl* __restrict__ ctrl) {
+                                                         const Ctrl* __restrict__ ctrl, uint32_t stage) {
     if (ctrl->halted) return;
     __shared__ uint32_t s_lo, s_hi;
     __shared__ unsigned long long s_c[kBlock];
@@ -224,7 +224,7 @@ __global__ void __launch_bounds__(kBlock) resort_scatter(const uint32_t* __restr
     }
     __syncthreads();
     const uint32_t lo = s_lo, cnt = s_hi - s_lo;
-    const bool staged = cnt <= kBlock;
+    const bool staged = cnt <= stage;  // stage <= kBlock: the composites fit s_c
     if (staged && tid < cnt) s_c[tid] = chg[lo + tid];
     __syncthreads();
     if (s < n) {

@@ -112,6 +112,7 @@ struct rrsph_ctx {
     bool use_graph = true;
     bool uploaded = false;
     bool stepped = false;
+    bool fields_valid = false;  // der / xs hold the last step's fields in A's slot order
     uint32_t passes = 0;
     uint32_t ntiles = 0;
     // buffers
@@ -119,6 +120,7 @@ struct rrsph_ctx {
     void *vaA = nullptr, *vaB = nullptr, *adA = nullptr, *pred = nullptr, *der = nullptr, *xs = nullptr,
          *pred_scratch = nullptr, *pres = nullptr, *pres_scratch = nullptr;
     void *pw = nullptr, *pw_scratch = nullptr;  // 3D: p / rho^2 (2D keeps it in pred.w)
+    void* adB = nullptr;  // A.ad of the step's input, kept only if a fused step may be undone
     bool fast_pair = false;
     int pair_impl = 1;  // 0 general, 1 two-phase walk with LDS hit lists, 2 LDS tiles, 3 single-phase walk
     uint32_t *oidA = nullptr, *oidB = nullptr;
@@ -150,6 +152,7 @@ struct rrsph_ctx {
     unsigned long long* chg_mask = nullptr;  // per wavefront of slots: ballot of the changed ones
     uint32_t* wpre = nullptr;                // per wavefront: changed slots in the wavefronts before
     uint32_t resort_cap = kResortCap;
+    uint32_t resort_stage = kBlock;     // changed composites a scatter block stages in LDS (more: binary search)
     bool incremental = true;            // false: every step runs the full radix sort
     bool need_full = true;              // skeyA does not describe A (after upload)
     uint64_t launched = 0;              // steps launched by this context
@@ -179,14 +182,22 @@ template <class R> StateA<R> stateA(rrsph_ctx* c) {
     return StateA<R>{c->posA, (real4<R>*)c->vaA, (real4<R>*)c->adA, c->oidA};
 }
 template <class R> StateB<R> stateB(rrsph_ctx* c, void* pred, void* pres) {
-    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB, (R*)pres};
+    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, c->oidB, (R*)pres, nullptr};
+}
+// the step's B copy; keeps A.ad too when the fused pass may have to be undone
+template <class R> StateB<R> stepB(rrsph_ctx* c) {
+    return StateB<R>{c->posB, (real4<R>*)c->vaB, (real4<R>*)c->pred, c->oidB, (R*)c->pres, (real4<R>*)c->adB};
 }
 // 3D: the position buffers hold double4 (two double2 slots per particle)
 template <class R> StateA3<R> stateA3(rrsph_ctx* c) {
     return StateA3<R>{(double4*)c->posA, (real4<R>*)c->vaA, (real4<R>*)c->adA, c->oidA};
 }
 template <class R> StateB3<R> stateB3(rrsph_ctx* c, void* pred, void* pw, void* pres) {
-    return StateB3<R>{(double4*)c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, (R*)pw, (R*)pres, c->oidB};
+    return StateB3<R>{(double4*)c->posB, (real4<R>*)c->vaB, (real4<R>*)pred, (R*)pw, (R*)pres, c->oidB, nullptr};
+}
+template <class R> StateB3<R> stepB3(rrsph_ctx* c) {
+    return StateB3<R>{(double4*)c->posB, (real4<R>*)c->vaB, (real4<R>*)c->pred, (R*)c->pw, (R*)c->pres, c->oidB,
+                      (real4<R>*)c->adB};
 }
 
 uint32_t grid_blocks(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
@@ -251,7 +262,7 @@ void enqueue_resort(rrsph_ctx* c, Launcher& L) {
     });
     L(K_RESORT_SCATTER, [&] {
         resort_scatter<<<nb, kBlock, 0, c->stream>>>(c->skeyA, c->chg, c->chg_mask, c->wpre, n, c->kbuf[0],
-                                                     c->vbuf[0], c->ctrl);
+                                                     c->vbuf[0], c->ctrl, c->resort_stage);
     });
     c->sorted_keys = c->kbuf[0];
     c->sorted_perm = c->vbuf[0];
@@ -269,7 +280,7 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof, bool full_s
         enqueue_resort(c, L);
     }
     StateA<R> A = stateA<R>(c);
-    StateB<R> B = stateB<R>(c, c->pred, c->pres);
+    StateB<R> B = stepB<R>(c);
     L(K_REORDER, [&] {
         reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
                                                         c->maxslots, c->skeyA);
@@ -326,7 +337,7 @@ template <class R> void enqueue_step3_t(rrsph_ctx* c, Profiler* prof, bool full_
         enqueue_resort(c, L);
     }
     StateA3<R> A = stateA3<R>(c);
-    StateB3<R> B = stateB3<R>(c, c->pred, c->pw, c->pres);
+    StateB3<R> B = stepB3<R>(c);
     L(K_REORDER, [&] {
         reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
                                                          c->maxslots, c->skeyA);
@@ -417,6 +428,46 @@ template <class R> void enqueue_util_reorder(rrsph_ctx* c) {
                                                     c->maxslots + 2 * kMaxSlots, nullptr);
 }
 
+// Up to kCopyRanges device-to-device copies in one launch (the saved-state
+// blobs of the adjoint move five arrays at a time; separate copies cost a
+// launch each, which dominates at tens of thousands of particles).  Lengths
+// are in 4-byte words; a range moves 16 bytes per access when both ends are
+// 16-byte aligned.
+constexpr int kCopyRanges = 8;
+struct CopyRange {
+    const uint32_t* src;
+    uint32_t* dst;
+    size_t words;
+};
+struct CopyPlan {
+    CopyRange r[kCopyRanges];
+    int n = 0;
+    void add(const void* src, void* dst, size_t bytes) { r[n++] = {(const uint32_t*)src, (uint32_t*)dst, bytes / 4}; }
+};
+__global__ void __launch_bounds__(kBlock) copy_ranges_kernel(CopyPlan plan) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (int k = 0; k < plan.n; ++k) {
+        const CopyRange c = plan.r[k];
+        size_t done = 0;
+        if (((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst)) & 15u) == 0) {
+            const size_t n4 = c.words / 4;
+            const uint4* __restrict__ s = (const uint4*)c.src;
+            uint4* __restrict__ d = (uint4*)c.dst;
+            for (size_t i = t; i < n4; i += stride) d[i] = s[i];
+            done = n4 * 4;
+        }
+        for (size_t i = done + t; i < c.words; i += stride) c.dst[i] = c.src[i];
+    }
+}
+rrsph_status enqueue_copies(rrsph_ctx* ctx, const CopyPlan& plan) {
+    size_t units = 0;
+    for (int k = 0; k < plan.n; ++k) units = std::max(units, plan.r[k].words / 4 + 1);
+    const uint32_t blocks = (uint32_t)std::min<size_t>(2048, (units + kBlock - 1) / kBlock);
+    copy_ranges_kernel<<<std::max(blocks, 1u), kBlock, 0, ctx->stream>>>(plan);
+    HIP_TRY(hipGetLastError());
+    return RRSPH_OK;
+}
+
 std::string status_message(uint32_t st) {
     switch (st) {
     case RRSPH_ERR_TOO_MANY_NEIGHBOURS: return "making neighbours grid: error occured";
@@ -426,12 +477,39 @@ std::string status_message(uint32_t st) {
     }
 }
 
+void enqueue_state_keys(rrsph_ctx* ctx);
+
+// A fused step whose neighbour lists overflowed under the consistency check
+// was integrated in its pair pass; the prologue after it refused to commit
+// it.  Put back the state the step started from: its sorted copy B (pos, va,
+// oid) and the copy of A.ad the reorder kept, then the cell keys of that
+// state for the next sort.
+rrsph_status rollback_fused_step(rrsph_ctx* ctx) {
+    const size_t n = ctx->P.n, pb = ctx->P.dim == 3 ? 32 : 16, vb = ctx->fp32 ? 16 : 32;
+    if (!ctx->adB) return fail(ctx, RRSPH_ERR_HIP, "RRSPH_HIP: no copy of the step's input to restore");
+    CopyPlan plan;
+    plan.add(ctx->posB, ctx->posA, pb * n);
+    plan.add(ctx->vaB, ctx->vaA, vb * n);
+    plan.add(ctx->adB, ctx->adA, vb * n);
+    plan.add(ctx->oidB, ctx->oidA, 4 * n);
+    if (const rrsph_status st = enqueue_copies(ctx, plan); st != RRSPH_OK) return st;
+    enqueue_state_keys(ctx);
+    ctx->need_full = true;
+    ctx->host_ctrl->rollback = 0;
+    HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return RRSPH_OK;
+}
+
 // Read the control block after the stream drained and turn it into a status.
 rrsph_status sync_status(rrsph_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const Ctrl& c = *ctx->host_ctrl;
     if (c.halted && c.status != 0 && c.status != kStatusResort) {
+        if (c.rollback) {
+            if (const rrsph_status st = rollback_fused_step(ctx); st != RRSPH_OK) return st;
+        }
         return fail(ctx, (rrsph_status)c.status, status_message(c.status));
     }
     if (c.pending) {
@@ -582,6 +660,10 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
         const long v = std::strtol(e, nullptr, 10);
         if (v >= 0 && v < (long)kResortCap) ctx->resort_cap = (uint32_t)v;
     }
+    if (const char* e = std::getenv("RRSPH_HIP_RESORT_STAGE")) {  // test hook: force the binary-search placement
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 0 && v <= (long)kBlock) ctx->resort_stage = (uint32_t)v;
+    }
     ctx->fused = !(p.dt_correction_method == 2 && p.dt_dynamic_mode == 1);
     if (const char* e = std::getenv("RRSPH_HIP_PAIR_KERNEL")) {
         if (!std::strcmp(e, "general")) ctx->pair_impl = 0;
@@ -619,11 +701,24 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch};
+                    c->pw_scratch, c->adB};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Cell keys of state A for the next step's sort.
+void enqueue_state_keys(rrsph_ctx* ctx) {
+    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    if (ctx->P.dim == 3)
+        state_keys3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->key_next);
+    else if (ctx->fp32)
+        state_keys_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const float4*)ctx->vaA,
+                                                                 ctx->key_next);
+    else
+        state_keys_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
+                                                                  ctx->key_next);
 }
 
 }  // namespace
@@ -698,6 +793,9 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
         ctx->pw = dalloc<char>(n * (r4 / 4), e);
         ctx->pw_scratch = dalloc<char>(n * (r4 / 4), e);
     }
+    // the fused pair pass integrates in the same launch; with the consistency
+    // check a neighbour overflow undoes the step from B and this copy of A.ad
+    if (ctx->fused && ctx->P.consistency_check) ctx->adB = dalloc<char>(n * r4, e);
     ctx->oidA = dalloc<uint32_t>(n, e);
     ctx->oidB = dalloc<uint32_t>(n, e);
     ctx->key_next = dalloc<uint32_t>(n, e);
@@ -793,6 +891,7 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->uploaded = true;
     ctx->stepped = false;
+    ctx->fields_valid = false;
     ctx->need_full = true;
     // step indices restart: forget the overflow history
     ctx->full_until = ctx->launched;
@@ -821,7 +920,10 @@ static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
         ctx->need_full = false;
     }
     HIP_TRY(hipGetLastError());
-    if (nsteps) ctx->stepped = true;
+    if (nsteps) {
+        ctx->stepped = true;
+        ctx->fields_valid = true;
+    }
     return RRSPH_OK;
 }
 
@@ -976,6 +1078,8 @@ rrsph_status rrsph_hip_set_export_fields(rrsph_ctx* ctx, int32_t enable) {
 
 rrsph_status rrsph_hip_fields(rrsph_ctx* ctx, double* a, double* drho, double* av, double* arvmu) {
     if (!ctx || !ctx->stepped) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no step taken yet");
+    if (!ctx->fields_valid)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: no step taken since the state was restored");
     if (ctx->fused && !ctx->export_fields)
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: enable rrsph_hip_set_export_fields before stepping");
     const size_t n = ctx->P.n, dim = ctx->P.dim;
@@ -1211,7 +1315,10 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     HIP_TRY(hipEventRecord(t1, ctx->stream));
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (nsteps) ctx->stepped = true;
+    if (nsteps) {
+        ctx->stepped = true;
+        ctx->fields_valid = true;
+    }
     std::vector<double> sum(K_NUM_CLASSES, 0.0);
     std::vector<uint32_t> cnt(K_NUM_CLASSES, 0);
     for (auto& r : prof.rec) {
@@ -1314,65 +1421,27 @@ static rrsph_status adjoint_supported(rrsph_ctx* ctx) {
     return RRSPH_OK;
 }
 
-// one saved state: posA | vaA | adA | oidA | ctrl
+// one saved state: posA | vaA | adA | oidA | pres | ctrl
 // bytes per particle of the FP64 position array (2D double2, 3D double4)
 // and of one real4 state array (va, ad: FP32 or FP64)
 static size_t pos_bytes(const rrsph_ctx* ctx) { return ctx->P.dim == 3 ? 32 : 16; }
 static size_t real4_bytes(const rrsph_ctx* ctx) { return ctx->fp32 ? 16 : 32; }
-// bytes of one saved state: pos | va | ad | oid | control block
+// bytes of one saved state: pos | va | ad | oid | pressure | control block
+// (the pressure of the step that produced the state, in its slot order: the
+// reference's p array, which download() returns)
 static size_t state_bytes(const rrsph_ctx* ctx) {
-    return (pos_bytes(ctx) + 2 * real4_bytes(ctx) + 4) * ctx->P.n + sizeof(Ctrl);
+    return (pos_bytes(ctx) + 2 * real4_bytes(ctx) + 4 + real4_bytes(ctx) / 4) * ctx->P.n + sizeof(Ctrl);
 }
 
-// Up to kCopyRanges device-to-device copies in one launch (the saved-state
-// blobs of the adjoint move five arrays at a time; separate copies cost a
-// launch each, which dominates at tens of thousands of particles).  Lengths
-// are in 4-byte words; a range moves 16 bytes per access when both ends are
-// 16-byte aligned.
-constexpr int kCopyRanges = 8;
-struct CopyRange {
-    const uint32_t* src;
-    uint32_t* dst;
-    size_t words;
-};
-struct CopyPlan {
-    CopyRange r[kCopyRanges];
-    int n = 0;
-    void add(const void* src, void* dst, size_t bytes) { r[n++] = {(const uint32_t*)src, (uint32_t*)dst, bytes / 4}; }
-};
-__global__ void __launch_bounds__(kBlock) copy_ranges_kernel(CopyPlan plan) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
-    for (int k = 0; k < plan.n; ++k) {
-        const CopyRange c = plan.r[k];
-        size_t done = 0;
-        if (((reinterpret_cast<uintptr_t>(c.src) | reinterpret_cast<uintptr_t>(c.dst)) & 15u) == 0) {
-            const size_t n4 = c.words / 4;
-            const uint4* __restrict__ s = (const uint4*)c.src;
-            uint4* __restrict__ d = (uint4*)c.dst;
-            for (size_t i = t; i < n4; i += stride) d[i] = s[i];
-            done = n4 * 4;
-        }
-        for (size_t i = done + t; i < c.words; i += stride) c.dst[i] = c.src[i];
-    }
-}
-static rrsph_status enqueue_copies(rrsph_ctx* ctx, const CopyPlan& plan) {
-    size_t units = 0;
-    for (int k = 0; k < plan.n; ++k) units = std::max(units, plan.r[k].words / 4 + 1);
-    const uint32_t blocks = (uint32_t)std::min<size_t>(2048, (units + kBlock - 1) / kBlock);
-    copy_ranges_kernel<<<std::max(blocks, 1u), kBlock, 0, ctx->stream>>>(plan);
-    HIP_TRY(hipGetLastError());
-    return RRSPH_OK;
-}
-
-// the blob of one saved state: posA | vaA | adA | oidA | ctrl
+// the blob of one saved state: posA | vaA | adA | oidA | pres | ctrl
 static CopyPlan state_plan(rrsph_ctx* ctx, void* blob, bool save) {
     const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
     char* d = (char*)blob;
-    void* arrays[5] = {ctx->posA, ctx->vaA, ctx->adA, ctx->oidA, ctx->ctrl};
-    const size_t bytes[5] = {pb * n, vb * n, vb * n, 4 * n, sizeof(Ctrl)};
+    void* arrays[6] = {ctx->posA, ctx->vaA, ctx->adA, ctx->oidA, ctx->pres, ctx->ctrl};
+    const size_t bytes[6] = {pb * n, vb * n, vb * n, 4 * n, vb / 4 * n, sizeof(Ctrl)};
     CopyPlan plan;
     size_t off = 0;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 6; ++k) {
         if (save) plan.add(arrays[k], d + off, bytes[k]);
         else plan.add(d + off, arrays[k], bytes[k]);
         off += bytes[k];
@@ -1385,18 +1454,11 @@ static rrsph_status save_state(rrsph_ctx* ctx, void* dst) { return enqueue_copie
 static rrsph_status restore_state(rrsph_ctx* ctx, const void* src, bool keys = true) {
     if (const rrsph_status st = enqueue_copies(ctx, state_plan(ctx, const_cast<void*>(src), false)); st != RRSPH_OK)
         return st;
+    ctx->fields_valid = false;  // der / xs belong to whatever step ran last
     // the next step's cell keys from the restored positions, and a full sort
     ctx->need_full = true;
     if (!keys) return RRSPH_OK;  // the caller reorders with a saved sort and launches no step
-    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
-    if (ctx->P.dim == 3)
-        state_keys3_kernel<<<nb, kBlock, 0, ctx->stream>>>(ctx->P, (const double4*)ctx->posA, ctx->key_next);
-    else if (ctx->fp32)
-        state_keys_kernel<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const float4*)ctx->vaA,
-                                                                 ctx->key_next);
-    else
-        state_keys_kernel<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P, ctx->posA, (const double4*)ctx->vaA,
-                                                                  ctx->key_next);
+    enqueue_state_keys(ctx);
     HIP_TRY(hipGetLastError());
     return RRSPH_OK;
 }

@@ -34,6 +34,7 @@ template <class R> struct StateB3 {
     R* pw;
     R* pres;
     uint32_t* oid;
+    real4<R>* ad = nullptr;  // copy of A.ad, kept when a fused step may have to be undone
 };
 
 __device__ __forceinline__ uint32_t cell_key3(const StepParams& P, double x, double y, double z, uint32_t& bad) {
@@ -89,6 +90,7 @@ __global__ void __launch_bounds__(kBlock) reorder3_kernel(StepParams P, const Ct
         B.pred[s] = pr;
         B.pw[s] = p / (pr.w * pr.w);
         B.pres[s] = p;
+        if (B.ad) B.ad[s] = ad;
         if (skey_keep) skey_keep[s] = k;
     }
     fill_cell_starts(P, s, live, k, skey, LB);

@@ -34,6 +34,7 @@ template <class R> struct StateB {
     real4<R>* pred;   // (vx^, vy^, rho^, p / rho^2)
     uint32_t* oid;
     R* pres;          // p, the reference's pressure array
+    real4<R>* ad = nullptr;  // copy of A.ad, kept when a fused step may have to be undone
 };
 
 // ------------------------------------------------------------------ grid keys
@@ -153,15 +154,22 @@ __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ct
     Ctrl& c = *ctrl;
     if (c.halted) return;
     if (c.pending) {
-        c.time += c.dt_new;
-        c.dt = c.dt_new;
-        c.step += 1;
-        c.pending = 0;
         if (P.consistency_check && c.nbr_overflow) {
+            // the reference throws in find_neighbours, before whole_step
+            // (GridFind.cpp:186-188): the step that overflowed is not
+            // committed, and the host puts back the state it started from
+            // (the fused pair pass has already integrated it)
+            c.pending = 0;
+            c.rollback = 1;
+            if (c.wall_disappear) c.nwm_done = 0;
             c.halted = 1;
             c.status = 3;  // RRSPH_ERR_TOO_MANY_NEIGHBOURS
             return;
         }
+        c.time += c.dt_new;
+        c.dt = c.dt_new;
+        c.step += 1;
+        c.pending = 0;
         if (P.consistency_check && c.outside > 0 && P.consistency_treatment == 1) {
             c.halted = 1;
             c.status = 5;  // RRSPH_ERR_OUTSIDE_BOUNDARIES
@@ -283,6 +291,7 @@ __global__ void __launch_bounds__(kBlock) reorder_kernel(StepParams P, const Ctr
     if (live) {
         B.pred[s] = pr;
         B.pres[s] = p;
+        if (B.ad) B.ad[s] = ad;
     }
     fill_cell_starts(P, s, live, k, skey, LB);
 }

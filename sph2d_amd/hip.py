@@ -233,6 +233,9 @@ class HipStep:
             raise RrsphError(st, msg)
         if not graph:
             self.L.rrsph_hip_set_graph(self.ctx, 0)
+        # counts the tapes adjoint_forward has recorded: a backward pass
+        # belongs to the tape of the forward it follows
+        self.tape_generation = 0
 
     def close(self):
         if getattr(self, "ctx", None) and self.ctx.value:
@@ -344,6 +347,7 @@ class HipStep:
         """Differentiable run from the uploaded state; gauges after each step,
         shape (nsteps, ngauges)."""
         eta = np.zeros((nsteps, spec.n))
+        self.tape_generation += 1
         self._check(self.L.rrsph_hip_adjoint_forward(self.ctx, int(nsteps), int(every), ctypes.byref(spec),
                                                      _dp(eta)))
         return eta

@@ -251,3 +251,58 @@ def test_adjoint_rejects_summation():
         sim.adjoint_forward(5, 5, spec)
     assert e.value.status == 7
     sim.close()
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_state_after_backward_is_the_forward_final_state(precision):
+    """After a reverse pass over several checkpoint segments the context
+    holds the forward's final state again: download() (pressure included)
+    and probe('p') return what they returned after the forward; fields()
+    refuses until a step has run, then works."""
+    lay, run = _case(nsteps=40, every=12)
+    run.sim.close()
+    sim = H.HipStep(lay.params, precision=precision)
+    sim.set_export_fields(True)
+    run.sim = sim
+    prm = lay.params
+    a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+    eta = run.forward(a, b)
+    pts = np.array([[0.3, 0.1], [0.6, 0.2], [0.9, 0.15]])
+    before, pb = sim.download(), sim.probe(pts, "p")
+    run.backward(_weights(eta.shape), len(a))
+    after, pa = sim.download(), sim.probe(pts, "p")
+    for k in ("r", "v", "rho", "p", "itype"):
+        assert np.array_equal(before[k], after[k]), k
+    assert np.array_equal(pb, pa)
+    with pytest.raises(H.RrsphError):
+        sim.fields()
+    sim.step(1)
+    assert np.all(np.isfinite(sim.fields()["a"]))
+    sim.close()
+
+
+def test_autograd_backward_after_another_forward_uses_its_own_tape():
+    """Two PistonGauges.apply calls on one context summed into one loss:
+    each backward gets the gradient of its own trajectory (the second
+    forward replaced the tape of the first, which is re-recorded)."""
+    import torch
+
+    lay, run = _case(nsteps=24, every=8)
+    prm = lay.params
+    a0, b0 = np.array(prm.piston_a), np.array(prm.piston_b)
+    a1, b1 = 1.5 * a0, -b0
+    c = _weights((24, 3))
+    ct = torch.from_numpy(c)
+    ta0 = torch.tensor(a0, requires_grad=True)
+    tb0 = torch.tensor(b0, requires_grad=True)
+    ta1 = torch.tensor(a1, requires_grad=True)
+    tb1 = torch.tensor(b1, requires_grad=True)
+    loss = (ct * PistonGauges.apply(ta0, tb0, run)).sum() + (ct * PistonGauges.apply(ta1, tb1, run)).sum()
+    loss.backward()
+    for a, b, ta, tb in ((a0, b0, ta0, tb0), (a1, b1, ta1, tb1)):
+        run.forward(a, b)
+        g = run.backward(c, len(a))
+        assert np.allclose(ta.grad.numpy(), g["a"], rtol=1e-12, atol=0)
+        assert np.allclose(tb.grad.numpy(), g["b"], rtol=1e-12, atol=0)
+    assert not np.allclose(ta0.grad.numpy(), ta1.grad.numpy())
+    run.sim.close()

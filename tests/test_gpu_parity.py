@@ -281,3 +281,18 @@ def test_incremental_resort_equals_full_sort(monkeypatch, precision):
     assert st["fallbacks"] > 0
     for k in ("r", "v", "rho", "p", "itype"):
         assert np.array_equal(full[k], tiny[k]), k
+
+
+def test_unstaged_resort_placement_equals_full_sort(monkeypatch):
+    """resort_scatter stages the changed composites that fall inside a
+    block's key range in LDS when there are at most 256 of them, and places
+    the block's slots by binary search over the global list otherwise.
+    RRSPH_HIP_RESORT_STAGE=0 forces the binary search on every block: 200
+    dam-break steps stay bit-identical to the full radix sort."""
+    lay = G.dam_break()
+    prm = lay.params
+    full, _ = _run_with_env(monkeypatch, {"RRSPH_HIP_SORT": "full"}, prm, lay.state, 200, H.FP32)
+    inc, si = _run_with_env(monkeypatch, {"RRSPH_HIP_RESORT_STAGE": "0"}, prm, lay.state, 200, H.FP32)
+    assert si["incremental"] == 199 and si["fallbacks"] == 0
+    for k in ("r", "v", "rho", "p", "itype"):
+        assert np.array_equal(full[k], inc[k]), k
