@@ -53,22 +53,32 @@ struct GaugeSpec {
 // ------------------------------------------------------------------ smooth gauges
 // Block partial sums (sum w e, sum w) of every gauge, gauge-major:
 // partials[(2 g) nb + b] = sum w e, partials[(2 g + 1) nb + b] = sum w of
-// block b (nb = gridDim.x).  A block with no particle inside any window
-// writes zeros without the per-gauge reductions (the sums are exactly zero).
+// block b (nb = gridDim.x).  The grid is at most kGaugeBlocks blocks; block
+// b sums the particle chunks b, b + nb, b + 2 nb, ... of kBlock particles.
+// A chunk with no particle inside any window adds nothing and skips the
+// per-gauge reductions (its sums are exactly zero).
+constexpr uint32_t kGaugeBlocks = 1024;
+
+struct GaugeLds {
+    double z[kBlock / kWave][kMaxGauges], w[kBlock / kWave][kMaxGauges];
+};
+
+__device__ __forceinline__ void gauge_lds_clear(const GaugeSpec& G, GaugeLds& L) {
+    for (uint32_t i = threadIdx.x; i < (kBlock / kWave) * G.n; i += kBlock) {
+        L.z[i / G.n][i % G.n] = 0.0;
+        L.w[i / G.n][i % G.n] = 0.0;
+    }
+    __syncthreads();
+}
+
+// One chunk: every thread of the block calls it (uniform control flow).
 template <class D2>
-__device__ __forceinline__ void gauge_block_sums(const GaugeSpec& G, bool in, double y, D2&& d2, double inv_r2,
-                                                 double* __restrict__ partials) {
-    __shared__ double s_z[kBlock / kWave][kMaxGauges], s_w[kBlock / kWave][kMaxGauges];
-    const uint32_t wave = threadIdx.x / kWave, nb = gridDim.x;
+__device__ __forceinline__ void gauge_chunk_sums(const GaugeSpec& G, bool in, double y, D2&& d2, double inv_r2,
+                                                 GaugeLds& L) {
+    const uint32_t wave = threadIdx.x / kWave;
     bool near = false;
     for (uint32_t g = 0; g < G.n; ++g) near |= in && d2(g) * inv_r2 < 1.0;
-    if (!__syncthreads_or(near)) {
-        for (uint32_t g = threadIdx.x; g < G.n; g += kBlock) {
-            partials[(size_t)(2 * g) * nb + blockIdx.x] = 0.0;
-            partials[(size_t)(2 * g + 1) * nb + blockIdx.x] = 0.0;
-        }
-        return;
-    }
+    if (!__syncthreads_or(near)) return;
     for (uint32_t g = 0; g < G.n; ++g) {
         const double u = 1.0 - d2(g) * inv_r2;
         const double w = in && u > 0.0 ? u * u * u : 0.0;
@@ -79,38 +89,53 @@ __device__ __forceinline__ void gauge_block_sums(const GaugeSpec& G, bool in, do
             z += __shfl_xor(z, off);
             ww += __shfl_xor(ww, off);
         }
-        if (__lane_id() == 0) {
-            s_z[wave][g] = z;
-            s_w[wave][g] = ww;
+        if (__lane_id() == 0) {  // each wave owns its row
+            L.z[wave][g] += z;
+            L.w[wave][g] += ww;
         }
     }
+}
+
+__device__ __forceinline__ void gauge_block_store(const GaugeSpec& G, GaugeLds& L, double* __restrict__ partials) {
     __syncthreads();
+    const uint32_t nb = gridDim.x;
     for (uint32_t g = threadIdx.x; g < G.n; g += kBlock) {
         double z = 0, ww = 0;
         for (uint32_t w = 0; w < kBlock / kWave; ++w) {
-            z += s_z[w][g];
-            ww += s_w[w][g];
+            z += L.z[w][g];
+            ww += L.w[w][g];
         }
         partials[(size_t)(2 * g) * nb + blockIdx.x] = z;
         partials[(size_t)(2 * g + 1) * nb + blockIdx.x] = ww;
     }
 }
 
+__host__ __device__ inline uint32_t gauge_blocks(uint32_t n) {
+    const uint32_t chunks = (n + kBlock - 1) / kBlock;
+    return chunks < kGaugeBlocks ? (chunks ? chunks : 1u) : kGaugeBlocks;
+}
+
 template <class R>
 __global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, const double2* __restrict__ pos,
                                                                 const real4<R>* __restrict__ va, GaugeSpec G,
                                                                 double* __restrict__ partials) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    bool in = false;
-    double x = 0, y = 0;
-    if (s < n) {
-        const double2 p = pos[s];
-        in = (int)va[s].w == G.ptype;
-        x = p.x;
-        y = p.y;
+    __shared__ GaugeLds L;
+    gauge_lds_clear(G, L);
+    const double inv_r2 = 1.0 / (G.radius * G.radius);
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t s = base + threadIdx.x;
+        bool in = false;
+        double x = 0, y = 0;
+        if (s < n) {
+            const double2 p = pos[s];
+            in = (int)va[s].w == G.ptype;
+            x = p.x;
+            y = p.y;
+        }
+        auto d2 = [&](uint32_t g) { return (x - G.x[g]) * (x - G.x[g]); };
+        gauge_chunk_sums(G, in, y, d2, inv_r2, L);
     }
-    auto d2 = [&](uint32_t g) { return (x - G.x[g]) * (x - G.x[g]); };
-    gauge_block_sums(G, in, y, d2, 1.0 / (G.radius * G.radius), partials);
+    gauge_block_store(G, L, partials);
 }
 
 // One block per gauge: each thread sums a fixed stride of the block

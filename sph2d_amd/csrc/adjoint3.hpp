@@ -37,18 +37,24 @@ __device__ __forceinline__ double gauge_weight3(const GaugeSpec3& G, uint32_t g,
 // gauge_block_sums).
 __global__ void __launch_bounds__(kBlock) smooth_gauge_partials3(uint32_t n, const double4* __restrict__ pos,
                                                                  GaugeSpec3 G, double* __restrict__ partials) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    bool in = false;
-    double4 p = make_double4(0, 0, 0, 0);
-    if (s < n) {
-        p = pos[s];
-        in = (int)p.w == G.g.ptype;
+    __shared__ GaugeLds L;
+    gauge_lds_clear(G.g, L);
+    const double inv_r2 = 1.0 / (G.g.radius * G.g.radius);
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t s = base + threadIdx.x;
+        bool in = false;
+        double4 p = make_double4(0, 0, 0, 0);
+        if (s < n) {
+            p = pos[s];
+            in = (int)p.w == G.g.ptype;
+        }
+        auto d2 = [&](uint32_t g) {
+            const double dx = p.x - G.g.x[g], dz = p.z - G.z[g];
+            return dx * dx + dz * dz;
+        };
+        gauge_chunk_sums(G.g, in, p.y, d2, inv_r2, L);
     }
-    auto d2 = [&](uint32_t g) {
-        const double dx = p.x - G.g.x[g], dz = p.z - G.z[g];
-        return dx * dx + dz * dz;
-    };
-    gauge_block_sums(G.g, in, p.y, d2, 1.0 / (G.g.radius * G.g.radius), partials);
+    gauge_block_store(G.g, L, partials);
 }
 
 // lam[oid].r += sum_g eta_bar[g] d eta_g / d r (state in sorted order).

@@ -1394,7 +1394,7 @@ static rrsph_status to_gauge_spec(rrsph_ctx* ctx, const rrsph_gauge_spec* in, Ga
 // smooth gauges of state A into eta/zs/ws (device, kMaxGauges entries each)
 static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, double* partials, double* eta,
                                           double* zs, double* ws) {
-    const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
+    const uint32_t nb = gauge_blocks(ctx->P.n);  // partials holds 2 kMaxGauges nb doubles
     if (ctx->P.dim == 3)
         smooth_gauge_partials3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, G, partials);
     else if (ctx->fp32)
