@@ -513,8 +513,18 @@ rrsph_status sync_status(rrsph_ctx* ctx) {
         return fail(ctx, (rrsph_status)c.status, status_message(c.status));
     }
     if (c.pending) {
-        if (ctx->P.consistency_check && c.nbr_overflow)
+        if (ctx->P.consistency_check && c.nbr_overflow) {
+            // the last launched step was a fused one that overflowed: no
+            // prologue ran after it, so do what it would have done (leave the
+            // step uncommitted, put its input back, stay halted)
+            Ctrl& h = *ctx->host_ctrl;
+            h.pending = 0;
+            h.halted = 1;
+            h.status = RRSPH_ERR_TOO_MANY_NEIGHBOURS;
+            if (h.wall_disappear) h.nwm_done = 0;
+            if (const rrsph_status st = rollback_fused_step(ctx); st != RRSPH_OK) return st;
             return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
+        }
         if (ctx->P.consistency_check && c.outside > 0 && ctx->P.consistency_treatment == 1)
             return fail(ctx, RRSPH_ERR_OUTSIDE_BOUNDARIES, status_message(RRSPH_ERR_OUTSIDE_BOUNDARIES));
     }
