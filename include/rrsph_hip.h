@@ -242,6 +242,18 @@ typedef struct rrsph_gauge_spec {
  * eta_g = y_ref + tau * log(sum_i w_i exp((y_i - y_ref) / tau) / sum_i w_i)
  * over particles of type ptype (0 where the window is empty). */
 rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta);
+/* rrsph_hip_smooth_gauges without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device or host memory (tight
+ * control loops reading the gauges every step, code that forbids hipMalloc):
+ * its block partial sums live in a fixed device array of the library
+ * (1.03 MiB per device, kept until the process ends, its address looked up
+ * once per device when a context is created) and the results pass through a
+ * pinned buffer the context allocated at creation.  At most
+ * RRSPH_MAX_GAUGES gauges (more: RRSPH_ERR_INVALID_ARGUMENT), any particle
+ * count.  Results equal rrsph_hip_smooth_gauges' bit for bit.  All contexts
+ * on one device share that array: calls that overlap in time on one device
+ * (other host threads, other streams) race; calls in sequence do not. */
+rrsph_status rrsph_hip_smooth_gauges_noalloc(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta);
 /* Replace the Fourier piston coefficients (nwm = 6):
  * x(t) = sum_k a_k sin(k w t) + b_k cos(k w t), k = 1 .. nharm; with
  * piston_segments paddles, a and b hold nharm harmonics per paddle,

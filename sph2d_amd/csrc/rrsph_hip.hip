@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,35 @@ double host_w(const rrsph_params_c& p, double dist, int skf) {
 
 }  // namespace
 
+// Device scratch of rrsph_hip_smooth_gauges_noalloc: the block partial sums
+// of every gauge (2 kMaxGauges kGaugeBlocks doubles), then eta, sum w e and
+// sum w of every gauge (3 kMaxGauges).  1.03 MiB per device, held until the
+// process ends.  Host code never uses this name as an address: the device's
+// own address comes from hipGetSymbolAddress (scratch_address below).
+constexpr size_t kScratchDoubles = 2 * (size_t)kMaxGauges * kGaugeBlocks + 3 * (size_t)kMaxGauges;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch;  // per device ordinal: the address of `scratch` on that device
+
+// Address of `scratch` on the current device, looked up once per device (at
+// context creation, with the context's device current).
+hipError_t scratch_address(int device, double** out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= g_scratch.size()) g_scratch.resize(device + 1, nullptr);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch[device] = (double*)p;
+    }
+    *out = g_scratch[device];
+    return hipSuccess;
+}
+}  // namespace
+
 // Checkpoints and per-step records of a differentiable run (FP64 only).
 struct AdjTape {
     bool valid = false;
@@ -132,6 +162,8 @@ struct rrsph_ctx {
     Ctrl* ctrl = nullptr;
     Ctrl* util_ctrl = nullptr;
     Ctrl* host_ctrl = nullptr;  // pinned mirror
+    double* scratch = nullptr;  // this device's `scratch` (rrsph_hip_smooth_gauges_noalloc)
+    double* host_eta = nullptr; // pinned, kMaxGauges doubles: gauge results on their way to the caller
     double* red = nullptr;      // diagnostics partials
     // staging for up/download (device)
     double *st_r = nullptr, *st_v = nullptr, *st_rho = nullptr, *st_p = nullptr, *st_a = nullptr,
@@ -715,6 +747,7 @@ void free_all(rrsph_ctx* c) {
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
+    if (c->host_eta) (void)hipHostFree(c->host_eta);
     if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -827,6 +860,8 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->P.chg_mask = ctx->chg_mask;
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&ctx->host_eta, kMaxGauges * sizeof(double), hipHostMallocDefault));
+    HIP_TRY(scratch_address(ctx->device, &ctx->scratch));
     std::memset(ctx->host_ctrl, 0, sizeof(Ctrl));
     HIP_TRY(hipMemcpy(ctx->util_ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice));
     return RRSPH_OK;
@@ -1571,6 +1606,28 @@ rrsph_status rrsph_hip_smooth_gauges(rrsph_ctx* ctx, const rrsph_gauge_spec* spe
     }
     HIP_TRY(hipFree(buf));
     return st;
+}
+
+rrsph_status rrsph_hip_smooth_gauges_noalloc(rrsph_ctx* ctx, const rrsph_gauge_spec* spec, double* eta) {
+    if (!ctx || !ctx->uploaded || !eta || !spec)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
+    if (spec->n > kMaxGauges)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT,
+                    "RRSPH_HIP: rrsph_hip_smooth_gauges_noalloc takes at most 64 gauges (its scratch holds 64)");
+    GaugeSpec3 G;
+    rrsph_status st = to_gauge_spec(ctx, spec, G);
+    if (st != RRSPH_OK) return st;
+    if (!ctx->scratch || !ctx->host_eta) return fail(ctx, RRSPH_ERR_HIP, "RRSPH_HIP: no scratch on this device");
+    // every partial and result slot used below is written before it is read:
+    // nothing in the scratch needs clearing
+    double* partials = ctx->scratch;
+    double* out = ctx->scratch + 2 * (size_t)kMaxGauges * kGaugeBlocks;
+    st = enqueue_smooth_gauges(ctx, G, partials, out, out + kMaxGauges, out + 2 * kMaxGauges);
+    if (st != RRSPH_OK) return st;
+    HIP_TRY(hipMemcpyAsync(ctx->host_eta, out, G.g.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    std::memcpy(eta, ctx->host_eta, G.g.n * sizeof(double));
+    return RRSPH_OK;
 }
 
 rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b) {

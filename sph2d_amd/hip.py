@@ -38,7 +38,7 @@ EXPORTED = [
     "rrsph_hip_fields", "rrsph_hip_cell_keys", "rrsph_hip_neighbours", "rrsph_hip_diagnostics", "rrsph_hip_gauges",
     "rrsph_hip_stream", "rrsph_hip_set_graph", "rrsph_hip_profile_steps", "rrsph_hip_kernel_class_count",
     "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats",
-    "rrsph_hip_smooth_gauges", "rrsph_hip_set_piston", "rrsph_hip_adjoint_forward", "rrsph_hip_adjoint_backward",
+    "rrsph_hip_smooth_gauges", "rrsph_hip_smooth_gauges_noalloc", "rrsph_hip_set_piston", "rrsph_hip_adjoint_forward", "rrsph_hip_adjoint_backward",
     "rrsph_hip_probe",
     "RRSPH_HIP_GetSpecificVersionMajor",
     "RRSPH_HIP_GetSpecificVersionMinor", "RRSPH_HIP_GetSpecificVersionPatch", "RRSPH_HIP_GetSpecificVersionName",
@@ -166,6 +166,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "rrsph_hip_sort_stats": (st, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64), up]),
         "rrsph_hip_smooth_gauges": (st, [vp, ctypes.POINTER(GaugeSpec), dp]),
+        "rrsph_hip_smooth_gauges_noalloc": (st, [vp, ctypes.POINTER(GaugeSpec), dp]),
         "rrsph_hip_set_piston": (st, [vp, ctypes.c_double, ctypes.c_uint32, dp, dp]),
         "rrsph_hip_adjoint_forward": (st, [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(GaugeSpec), dp]),
         "rrsph_hip_adjoint_backward": (st, [vp, dp, dp, dp, dp, dp, dp]),
@@ -331,6 +332,12 @@ class HipStep:
     def smooth_gauges(self, spec: GaugeSpec) -> np.ndarray:
         out = np.zeros(spec.n)
         self._check(self.L.rrsph_hip_smooth_gauges(self.ctx, ctypes.byref(spec), _dp(out)))
+        return out
+
+    def smooth_gauges_noalloc(self, spec: GaugeSpec) -> np.ndarray:
+        """smooth_gauges through the entry point that allocates nothing."""
+        out = np.zeros(spec.n)
+        self._check(self.L.rrsph_hip_smooth_gauges_noalloc(self.ctx, ctypes.byref(spec), _dp(out)))
         return out
 
     def set_piston(self, omega: float, a: Sequence[float], b: Sequence[float]):
