@@ -296,6 +296,16 @@ rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total
 rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, uint64_t* full_steps,
                                   uint64_t* fallbacks, uint32_t* last_changed);
 
+/* Neighbour-list statistics.  No reference counterpart (the reference
+ * builds its lists from scratch every step, GridFind.cpp:112-189): in the 2D
+ * cubic configuration the step sorts and walks the cells only on build
+ * steps, which list every particle within k h + skin, and the steps between
+ * test those lists with the exact cut-off.  Steps that read the lists,
+ * build steps, steps that halted because the lists might be stale (re-run
+ * as builds), and the current build interval (0: lists off). */
+rrsph_status rrsph_hip_list_stats(rrsph_ctx* ctx, uint64_t* list_steps, uint64_t* builds, uint64_t* stale_halts,
+                                  uint32_t* interval);
+
 /* Version hooks of the backend (RRSPH_COMMON/include/CommonIncl.h:14-17). */
 uint32_t RRSPH_HIP_GetSpecificVersionMajor(void);
 uint32_t RRSPH_HIP_GetSpecificVersionMinor(void);

@@ -65,6 +65,11 @@ struct Ctrl {
     uint32_t wall_disappear;
     uint32_t rollback;      // 1: the halted step was integrated by the fused pair pass; the host restores its input
     double piston_vseg[8];  // segmented piston: velocity of each paddle
+    // neighbour-list steps
+    double disp_acc;        // sum over the steps since the last list build of each step's largest displacement
+    double disp_last;       // the largest displacement of the last committed list or build step
+    uint32_t list_overflow; // the last build found more candidates for some particle than its list holds
+    uint32_t list_valid;    // the lists describe the current slot order (a build ran; no sort since)
 };
 
 constexpr int kMaxSegments = 8;  // paddles of a segmented piston
@@ -127,6 +132,12 @@ struct StepParams {
     // incremental re-sort: the integrator stores the ballot of the slots of
     // [64 w, 64 w + 64) whose particle moved to another cell in chg_mask[w]
     unsigned long long* chg_mask;
+    // neighbour lists: every particle within list_reach = k h + skin at the
+    // build (squared, with a rounding margin: list_r2), at most list_cap per
+    // particle; valid while the summed largest displacements stay below
+    // skin_half = skin / 2 (less a rounding margin)
+    double list_reach, list_r2, skin_half;
+    uint32_t list_cap;
 };
 
 // XCD-aware block index: the hardware deals consecutive workgroups to the 8

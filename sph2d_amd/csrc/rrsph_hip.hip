@@ -24,6 +24,7 @@
 #include "step_kernels.hpp"
 #include "step3d.hpp"
 #include "pair_tile.hpp"
+#include "list_step.hpp"
 
 using namespace rrsph;
 
@@ -40,11 +41,13 @@ enum KernelClass : uint32_t {
     K_INTEGRATE,
     K_RESORT_SORT,
     K_RESORT_SCATTER,
+    K_LIST_BUILD,
     K_NUM_CLASSES
 };
 const char* const kClassNames[K_NUM_CLASSES] = {"step_prologue", "radix_upsweep", "radix_scan", "radix_downsweep",
                                                 "reorder_kernel", "density_sum_kernel", "pair_kernel",
-                                                "integrate_kernel", "resort_sort_small", "resort_scatter"};
+                                                "integrate_kernel", "resort_sort_small", "resort_scatter",
+                                                "list_build_kernel"};
 
 struct Profiler {
     std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> rec;
@@ -193,6 +196,18 @@ struct rrsph_ctx {
     uint64_t full_run_end = 0;          // step index where the last run of full sorts ended
     uint64_t full_run = 0;              // length of the last run of full sorts
     uint64_t n_incremental = 0, n_full = 0, n_fallback = 0;
+    // neighbour-list steps (2D cubic configuration): lists built every
+    // list_interval steps (host schedule, adapted to the measured
+    // displacement rate); the device halts a step whose lists may be stale
+    bool list_mode = false;
+    bool list_suspended = false;        // the adjoint's tapes and replays run without lists
+    uint32_t *nbl = nullptr, *nbc = nullptr;
+    unsigned long long* dslots = nullptr;  // per-step largest displacement (block maxima)
+    hipGraph_t graph_list = nullptr, graph_build = nullptr;
+    hipGraphExec_t gexec_list = nullptr, gexec_build = nullptr;
+    uint64_t build_at = 0;              // step index of the last build launched
+    uint64_t list_interval = 8;         // steps between builds
+    uint64_t n_list = 0, n_build = 0, n_list_halts = 0;
     AdjTape adj;
 };
 
@@ -399,6 +414,54 @@ template <class R> void enqueue_step3_t(rrsph_ctx* c, Profiler* prof, bool full_
     }
 }
 
+// One neighbour-list step (build: sort, reorder and new lists first).
+template <class R> void enqueue_list_step_t(rrsph_ctx* c, Profiler* prof, bool build) {
+    Launcher L{c, prof};
+    const StepParams& P = c->P;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
+    L(K_PROLOGUE, [&] {
+        list_prologue<<<1, kBlock, 0, c->stream>>>(P, c->ctrl, c->dslots, c->maxslots, build ? 1u : 0u);
+    });
+    ListState<R> X{c->posA, (const real4<R>*)c->vaA, c->oidA};
+    if (build) {
+        enqueue_sort(c, c->ctrl, L);
+        StateA<R> A = stateA<R>(c);
+        StateB<R> B = stateB<R>(c, c->pred, c->pres);
+        L(K_REORDER, [&] {
+            reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
+                                                            c->maxslots, nullptr);
+        });
+        L(K_LIST_BUILD, [&] {
+            list_build_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, c->posB, (const real4<R>*)c->vaB, c->LB,
+                                                                   c->nbl, c->nbc);
+        });
+        X = ListState<R>{c->posB, (const real4<R>*)c->vaB, c->oidB};
+    }
+    real4<R>* pred = (real4<R>*)c->pred;
+    if (P.density_treatment == 0) {
+        L(K_DENSITY_SUM, [&] {
+            density_sum_list<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, pred, (R*)c->pres, c->nbl, c->nbc);
+        });
+    }
+    L(K_PAIR, [&] {
+        pair_list_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, pred, (R*)c->pres, c->nbl, c->nbc,
+                                                              (real4<R>*)c->der, (real2<R>*)c->xs, c->maxslots);
+    });
+    L(K_INTEGRATE, [&] {
+        integrate_list_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, X, (const real4<R>*)c->der,
+                                                               (const real2<R>*)c->xs, pred, stateA<R>(c),
+                                                               c->key_next, c->maxslots, c->dslots);
+    });
+}
+
+void enqueue_list_step(rrsph_ctx* c, Profiler* prof, bool build) {
+    if (c->fp32) enqueue_list_step_t<float>(c, prof, build);
+    else enqueue_list_step_t<double>(c, prof, build);
+}
+
+bool list_active(const rrsph_ctx* c) { return c->list_mode && !c->list_suspended; }
+
 void enqueue_step(rrsph_ctx* c, Profiler* prof, bool full_sort) {
     if (c->P.dim == 3) {
         if (c->fp32) enqueue_step3_t<float>(c, prof, full_sort);
@@ -410,14 +473,16 @@ void enqueue_step(rrsph_ctx* c, Profiler* prof, bool full_sort) {
 }
 
 void drop_graph(rrsph_ctx* c) {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    if (c->graph) (void)hipGraphDestroy(c->graph);
-    if (c->gexec_full) (void)hipGraphExecDestroy(c->gexec_full);
-    if (c->graph_full) (void)hipGraphDestroy(c->graph_full);
-    c->gexec = nullptr;
-    c->graph = nullptr;
-    c->gexec_full = nullptr;
-    c->graph_full = nullptr;
+    hipGraphExec_t* ex[] = {&c->gexec, &c->gexec_full, &c->gexec_list, &c->gexec_build};
+    hipGraph_t* gr[] = {&c->graph, &c->graph_full, &c->graph_list, &c->graph_build};
+    for (auto* e : ex) {
+        if (*e) (void)hipGraphExecDestroy(*e);
+        *e = nullptr;
+    }
+    for (auto* g : gr) {
+        if (*g) (void)hipGraphDestroy(*g);
+        *g = nullptr;
+    }
 }
 
 // Capture one step with the incremental re-sort and one with the full sort.
@@ -431,6 +496,16 @@ rrsph_status build_graph(rrsph_ctx* ctx) {
     enqueue_step(ctx, nullptr, true);
     HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph_full));
     HIP_TRY(hipGraphInstantiate(&ctx->gexec_full, ctx->graph_full, nullptr, nullptr, 0));
+    if (ctx->list_mode) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_list_step(ctx, nullptr, false);
+        HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph_list));
+        HIP_TRY(hipGraphInstantiate(&ctx->gexec_list, ctx->graph_list, nullptr, nullptr, 0));
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_list_step(ctx, nullptr, true);
+        HIP_TRY(hipStreamEndCapture(ctx->stream, &ctx->graph_build));
+        HIP_TRY(hipGraphInstantiate(&ctx->gexec_build, ctx->graph_build, nullptr, nullptr, 0));
+    }
     return RRSPH_OK;
 }
 
@@ -538,7 +613,8 @@ rrsph_status sync_status(rrsph_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const Ctrl& c = *ctx->host_ctrl;
-    if (c.halted && c.status != 0 && c.status != kStatusResort) {
+    const bool rerun = c.status == kStatusResort || c.status == kStatusRebuild || c.status == kStatusListOverflow;
+    if (c.halted && c.status != 0 && !rerun) {
         if (c.rollback) {
             if (const rrsph_status st = rollback_fused_step(ctx); st != RRSPH_OK) return st;
         }
@@ -716,6 +792,15 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&
                      p.average_velocity_skf == 1 && p.intf_sph_approximation == 2 && !p.artificial_pressure &&
                      (p.density_treatment == 0 || p.density_treatment == 1) && std::fabs(p.cell_scale_k - 2.0) == 0.0;
+    // neighbour-list steps: the 2D cubic configuration; skin as a fraction of the cut-off
+    double skin_frac = 0.125;
+    if (const char* e = std::getenv("RRSPH_HIP_LIST_SKIN")) skin_frac = std::strtod(e, nullptr);
+    ctx->list_mode = !d3 && ctx->fast_pair && skin_frac > 0.0 && skin_frac < 0.9;
+    if (const char* e = std::getenv("RRSPH_HIP_LIST")) ctx->list_mode = ctx->list_mode && std::strcmp(e, "0") != 0;
+    P.list_reach = P.cell * (1.0 + skin_frac);
+    P.list_r2 = P.list_reach * P.list_reach * (1.0 + 1e-12);
+    P.skin_half = 0.5 * P.cell * skin_frac * (1.0 - 1e-9);
+    P.list_cap = (uint32_t)std::ceil(p.max_neighbours * (1.0 + skin_frac) * (1.0 + skin_frac)) + 8;
     if (d3 && !ctx->fast_pair)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: 3D supports cubic kernels for every term, approximation 2, no artificial pressure, "
@@ -743,7 +828,7 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch, c->adB};
+                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->dslots};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -858,7 +943,13 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->chg_mask = dalloc<unsigned long long>(n / kWave + 1, e);
     ctx->wpre = dalloc<uint32_t>(n / kWave + 1, e);
     ctx->P.chg_mask = ctx->chg_mask;
+    if (ctx->list_mode) {
+        ctx->nbl = dalloc<uint32_t>((size_t)ctx->P.list_cap * n, e);
+        ctx->nbc = dalloc<uint32_t>(n, e);
+        ctx->dslots = dalloc<unsigned long long>(2 * kMaxSlots, e);
+    }
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (ctx->dslots) HIP_TRY(hipMemset(ctx->dslots, 0, 2 * kMaxSlots * sizeof(unsigned long long)));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_eta, kMaxGauges * sizeof(double), hipHostMallocDefault));
     HIP_TRY(scratch_address(ctx->device, &ctx->scratch));
@@ -947,10 +1038,42 @@ rrsph_status rrsph_hip_upload(rrsph_ctx* ctx, const double* r, const double* v, 
 
 // Enqueue nsteps steps: the first after an upload (or a re-sort overflow)
 // with the full radix sort, the others with the incremental re-sort.
+static uint64_t committed_steps(const Ctrl& c);
+
+// Whether the next list step is a build: after an upload, a restore or a
+// run without lists, and every list_interval steps.
+static bool next_is_build(const rrsph_ctx* ctx, uint64_t step) {
+    return ctx->need_full || step - ctx->build_at >= ctx->list_interval;
+}
+
 static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
     if (ctx->use_graph && !ctx->gexec) {
         rrsph_status st = build_graph(ctx);
         if (st != RRSPH_OK) return st;
+    }
+    if (list_active(ctx)) {
+        // step indices from the committed count of the last synchronisation
+        const uint64_t base = committed_steps(*ctx->host_ctrl);
+        for (uint32_t k = 0; k < nsteps; ++k) {
+            const bool build = next_is_build(ctx, base + k);
+            if (ctx->use_graph) HIP_TRY(hipGraphLaunch(build ? ctx->gexec_build : ctx->gexec_list, ctx->stream));
+            else enqueue_list_step(ctx, nullptr, build);
+            if (build) {
+                ctx->build_at = base + k;
+                ++ctx->n_build;
+                ++ctx->n_full;
+            } else {
+                ++ctx->n_list;
+            }
+            ++ctx->launched;
+            ctx->need_full = false;
+        }
+        HIP_TRY(hipGetLastError());
+        if (nsteps) {
+            ctx->stepped = true;
+            ctx->fields_valid = true;
+        }
+        return RRSPH_OK;
     }
     for (uint32_t k = 0; k < nsteps; ++k) {
         const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
@@ -993,33 +1116,76 @@ static void note_resort_overflow(rrsph_ctx* ctx, uint64_t at) {
     ctx->need_full = true;  // the overflowing step itself runs with the full sort
 }
 
-// After a synchronised launch: if a step found more changed slots than the
-// incremental re-sort takes, it halted before touching the state; clear the
-// halt and re-run it (and the steps after it) starting with the full sort.
-static rrsph_status resume_after_resort(rrsph_ctx* ctx, uint64_t first, uint32_t nsteps, uint32_t* left) {
+// A list step found its lists possibly stale at step index `at` (the
+// displacements since the build reached half the skin): build every
+// 0.8 x that many steps from now on, until the measured rate says more.
+static void note_list_stale(rrsph_ctx* ctx, uint64_t at) {
+    ++ctx->n_list_halts;
+    const uint64_t age = at > ctx->build_at ? at - ctx->build_at : 1;
+    ctx->list_interval = std::max<uint64_t>(1, age * 4 / 5);
+    ctx->need_full = true;  // the halted step runs as a build
+}
+
+// After a clean batch: the interval between builds from the displacement
+// per step (80% of the steps that use up half the skin), the larger of the
+// mean since the last build and the last step's.
+static void adapt_list_interval(rrsph_ctx* ctx) {
+    const Ctrl& c = *ctx->host_ctrl;
+    const uint64_t now = committed_steps(c);
+    // disp_acc covers the steps after the build up to the one before the last
+    const uint64_t counted = now > ctx->build_at + 1 ? now - ctx->build_at - 1 : 0;
+    double rate = c.disp_last;
+    if (counted > 0) rate = std::max(rate, c.disp_acc / (double)counted);
+    const double k = rate > 0.0 ? 0.8 * ctx->P.skin_half / rate : 1e9;
+    ctx->list_interval = (uint64_t)std::min(2048.0, std::max(1.0, std::floor(k)));
+}
+
+// After a synchronised launch: a step that halted for a reason the host
+// handles (more changed cells than the incremental re-sort takes; lists
+// possibly stale; a list out of room) did so before touching the state;
+// clear the halt and set up the re-run of that step.
+static rrsph_status resume_halt(rrsph_ctx* ctx, bool* resumed) {
     Ctrl& c = *ctx->host_ctrl;
-    *left = 0;
-    if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
-    const uint64_t at = committed_steps(c);  // the step that overflowed
-    note_resort_overflow(ctx, at);
-    *left = nsteps - (uint32_t)(at - first);
+    *resumed = false;
+    if (!c.halted) return RRSPH_OK;
+    const uint64_t at = committed_steps(c);  // the step that halted
+    if (c.status == kStatusResort) {
+        note_resort_overflow(ctx, at);
+    } else if (c.status == kStatusRebuild) {
+        note_list_stale(ctx, at);
+    } else if (c.status == kStatusListOverflow) {
+        ctx->list_mode = false;  // more candidates than a list holds: sort every step from here
+        ctx->need_full = true;
+    } else {
+        return RRSPH_OK;
+    }
     c.halted = 0;
     c.status = 0;
+    c.list_overflow = 0;
     HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+    *resumed = true;
     return RRSPH_OK;
 }
 
 // Run nsteps steps to completion (or to a halt) and report the status.
+// List steps go in batches of about twice the build interval, so that a
+// halt wastes few launches and the interval follows the flow.
 static rrsph_status run_steps(rrsph_ctx* ctx, uint32_t nsteps) {
-    const uint64_t first = committed_steps(*ctx->host_ctrl);
-    uint32_t left = nsteps;
+    const uint64_t target = committed_steps(*ctx->host_ctrl) + nsteps;
     while (true) {
-        rrsph_status st = launch_steps(ctx, left);
+        const uint64_t now = committed_steps(*ctx->host_ctrl);
+        if (now >= target) return RRSPH_OK;
+        uint64_t chunk = target - now;
+        if (list_active(ctx)) chunk = std::min<uint64_t>(chunk, std::min<uint64_t>(256, std::max<uint64_t>(16, 2 * ctx->list_interval)));
+        rrsph_status st = launch_steps(ctx, (uint32_t)chunk);
         if (st != RRSPH_OK) return st;
         st = sync_status(ctx);
         if (st != RRSPH_OK) return st;
-        st = resume_after_resort(ctx, first, nsteps, &left);
-        if (st != RRSPH_OK || !left) return st;
+        bool resumed = false;
+        st = resume_halt(ctx, &resumed);
+        if (st != RRSPH_OK) return st;
+        if (!resumed && ctx->host_ctrl->halted) return RRSPH_OK;  // stopped at t_end
+        if (!resumed && list_active(ctx)) adapt_list_interval(ctx);
     }
 }
 
@@ -1047,6 +1213,7 @@ rrsph_status rrsph_hip_run_until(rrsph_ctx* ctx, double t_end, uint64_t max_step
     }
     const uint64_t step1 = ctx->host_ctrl->step + (ctx->host_ctrl->pending ? 1 : 0);
     if (steps_taken) *steps_taken = step1 - step0;
+    if (list_active(ctx)) ctx->need_full = true;  // a list build launched after the stop did not run
     // clear the stop so later calls run
     ctx->host_ctrl->t_end = INFINITY;
     if (ctx->host_ctrl->halted && ctx->host_ctrl->status == 0) ctx->host_ctrl->halted = 0;
@@ -1351,9 +1518,21 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     HIP_TRY(hipEventRecord(t0, ctx->stream));
     const uint64_t first = committed_steps(*ctx->host_ctrl);
     for (uint32_t k = 0; k < nsteps; ++k) {
-        const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
-        enqueue_step(ctx, &prof, full);
-        ++(full ? ctx->n_full : ctx->n_incremental);
+        if (list_active(ctx)) {
+            const bool build = next_is_build(ctx, first + k);
+            enqueue_list_step(ctx, &prof, build);
+            if (build) {
+                ctx->build_at = first + k;
+                ++ctx->n_build;
+                ++ctx->n_full;
+            } else {
+                ++ctx->n_list;
+            }
+        } else {
+            const bool full = ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until;
+            enqueue_step(ctx, &prof, full);
+            ++(full ? ctx->n_full : ctx->n_incremental);
+        }
         ++ctx->launched;
         ctx->need_full = false;
     }
@@ -1382,10 +1561,11 @@ rrsph_status rrsph_hip_profile_steps(rrsph_ctx* ctx, uint32_t nsteps, double* av
     (void)hipEventDestroy(t1);
     rrsph_status st = sync_status(ctx);
     if (st != RRSPH_OK) return st;
-    uint32_t left = 0;
-    st = resume_after_resort(ctx, first, nsteps, &left);
-    if (st != RRSPH_OK || !left) return st;
-    return run_steps(ctx, left);  // finish the steps after a re-sort overflow, unprofiled
+    bool resumed = false;
+    st = resume_halt(ctx, &resumed);
+    if (st != RRSPH_OK || !resumed) return st;
+    // finish the steps after a halted one, unprofiled
+    return run_steps(ctx, (uint32_t)(first + nsteps - committed_steps(*ctx->host_ctrl)));
 }
 
 rrsph_status rrsph_hip_time_steps(rrsph_ctx* ctx, uint32_t nsteps, double* total_ms) {
@@ -1416,6 +1596,16 @@ rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, u
     if (full_steps) *full_steps = ctx->n_full;
     if (fallbacks) *fallbacks = ctx->n_fallback;
     if (last_changed) *last_changed = ctx->host_ctrl ? ctx->host_ctrl->resort_count : 0;
+    return RRSPH_OK;
+}
+
+rrsph_status rrsph_hip_list_stats(rrsph_ctx* ctx, uint64_t* list_steps, uint64_t* builds, uint64_t* stale_halts,
+                                  uint32_t* interval) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    if (list_steps) *list_steps = ctx->n_list;
+    if (builds) *builds = ctx->n_build;
+    if (stale_halts) *stale_halts = ctx->n_list_halts;
+    if (interval) *interval = ctx->list_mode ? (uint32_t)ctx->list_interval : 0u;
     return RRSPH_OK;
 }
 
@@ -1651,8 +1841,43 @@ rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, 
     return RRSPH_OK;
 }
 
+// The adjoint's tapes and replays run the steps without neighbour lists:
+// a replay must repeat its step exactly, whatever the list schedule was.
+// Entering and leaving that state forces a full sort and a list build.
+struct ListSuspend {
+    rrsph_ctx* c;
+    bool was;
+    explicit ListSuspend(rrsph_ctx* ctx) : c(ctx), was(ctx->list_suspended) {
+        if (c->list_mode && !was) c->need_full = true;
+        c->list_suspended = true;
+    }
+    ~ListSuspend() {
+        c->list_suspended = was;
+        if (c->list_mode && !was) c->need_full = true;
+    }
+};
+
+static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
+                                         const rrsph_gauge_spec* spec, double* eta);
+static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
+                                          double* grad_r0, double* grad_v0, double* grad_rho0);
+
 rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                                        const rrsph_gauge_spec* spec, double* eta) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    ListSuspend guard(ctx);
+    return adjoint_forward_impl(ctx, nsteps, checkpoint_every, spec, eta);
+}
+
+rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
+                                        double* grad_r0, double* grad_v0, double* grad_rho0) {
+    if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    ListSuspend guard(ctx);
+    return adjoint_backward_impl(ctx, eta_bar, grad_a, grad_b, grad_r0, grad_v0, grad_rho0);
+}
+
+static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
+                                         const rrsph_gauge_spec* spec, double* eta) {
     if (!ctx || !ctx->uploaded || nsteps == 0 || checkpoint_every == 0 || !eta)
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     rrsph_status st = adjoint_supported(ctx);
@@ -1735,8 +1960,8 @@ static void enqueue_gauge_grad(rrsph_ctx* ctx, uint32_t k, double* lam) {
                                                                   T.ws + o, lam);
 }
 
-rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
-                                        double* grad_r0, double* grad_v0, double* grad_rho0) {
+static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
+                                          double* grad_r0, double* grad_v0, double* grad_rho0) {
     if (!ctx || !eta_bar) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: bad arguments");
     AdjTape& T = ctx->adj;
     if (!T.valid) return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: run rrsph_hip_adjoint_forward first");

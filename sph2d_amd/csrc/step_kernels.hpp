@@ -153,6 +153,7 @@ __device__ __forceinline__ void slots_max2(const unsigned long long* __restrict_
 __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ctrl) {
     Ctrl& c = *ctrl;
     if (c.halted) return;
+    c.list_valid = 0;  // a sorted step reorders the slots; a list build sets it again
     if (c.pending) {
         if (P.consistency_check && c.nbr_overflow) {
             // the reference throws in find_neighbours, before whole_step
@@ -518,13 +519,17 @@ __global__ void __launch_bounds__(kBlock) density_sum_kernel(StepParams P, Ctrl*
 
 // ------------------------------------------------------------------ whole_step
 // WholeStep.cpp:19-61 for one particle, with the RRSPH_CL boundary motion
-// (cl/TimeIntegration.cl:190-193) and wave maker (:196-255) when enabled.
+// (cl/TimeIntegration.cl:190-193) and wave maker (:196-255) when enabled:
+// the new position and (v, rho, itype); `outside` counts a particle that
+// ends outside the geometry (ConsistencyCheck.cpp:162-189).
+template <class R> struct Stepped2 {
+    double2 pos;
+    real4<R> va;
+};
 template <class R>
-__device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c, uint32_t s, double2 pos,
-                                              real4<R> va, const real4<R>& pred, uint32_t oid, R ax, R ay, R drho,
-                                              R avx, R avy, double dt, StateA<R> A, uint32_t* __restrict__ key_next,
-                                              uint32_t key_old, uint32_t& outside, uint32_t& bad,
-                                              uint32_t& changed) {
+__device__ __forceinline__ Stepped2<R> whole_step_one(const StepParams& P, const Ctrl& c, double2 pos, real4<R> va,
+                                                      const real4<R>& pred, uint32_t oid, R ax, R ay, R drho, R avx,
+                                                      R avy, double dt, uint32_t& outside) {
     const double vdt_d = c.cur_first ? dt * 0.5 : dt;
     const R vdt = (R)vdt_d;
     int itype = (int)va.w;
@@ -561,13 +566,25 @@ __device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c
     }
     const bool inside = pos.x < P.xmax && pos.x > P.xmin && pos.y < P.ymax && pos.y > P.ymin;
     if (!inside) outside += 1;  // NaN positions fail the comparisons too
-    A.pos[s] = pos;
-    real4<R> nva;
-    nva.x = vx;
-    nva.y = vy;
-    nva.z = rho;
-    nva.w = (R)itype;
-    A.va[s] = nva;
+    Stepped2<R> o;
+    o.pos = pos;
+    o.va.x = vx;
+    o.va.y = vy;
+    o.va.z = rho;
+    o.va.w = (R)itype;
+    return o;
+}
+
+// whole_step of slot s into state A, and the slot's key for the next sort.
+template <class R>
+__device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c, uint32_t s, double2 pos,
+                                              real4<R> va, const real4<R>& pred, uint32_t oid, R ax, R ay, R drho,
+                                              R avx, R avy, double dt, StateA<R> A, uint32_t* __restrict__ key_next,
+                                              uint32_t key_old, uint32_t& outside, uint32_t& bad,
+                                              uint32_t& changed) {
+    const Stepped2<R> o = whole_step_one<R>(P, c, pos, va, pred, oid, ax, ay, drho, avx, avy, dt, outside);
+    A.pos[s] = o.pos;
+    A.va[s] = o.va;
     real4<R> nad;
     nad.x = ax;
     nad.y = ay;
@@ -575,7 +592,7 @@ __device__ __forceinline__ void integrate_one(const StepParams& P, const Ctrl& c
     nad.w = R(0);
     A.ad[s] = nad;
     A.oid[s] = oid;
-    const uint32_t key = itype == 0 ? P.ncells : cell_key(P, pos.x, pos.y, bad);
+    const uint32_t key = (int)o.va.w == 0 ? P.ncells : cell_key(P, o.pos.x, o.pos.y, bad);
     key_next[s] = key;
     changed = key != key_old ? 1u : 0u;
 }

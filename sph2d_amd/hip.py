@@ -37,7 +37,7 @@ EXPORTED = [
     "rrsph_hip_run_until", "rrsph_hip_download", "rrsph_hip_get_time", "rrsph_hip_set_export_fields",
     "rrsph_hip_fields", "rrsph_hip_cell_keys", "rrsph_hip_neighbours", "rrsph_hip_diagnostics", "rrsph_hip_gauges",
     "rrsph_hip_stream", "rrsph_hip_set_graph", "rrsph_hip_profile_steps", "rrsph_hip_kernel_class_count",
-    "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats",
+    "rrsph_hip_kernel_class_name", "rrsph_hip_time_steps", "rrsph_hip_sort_stats", "rrsph_hip_list_stats",
     "rrsph_hip_smooth_gauges", "rrsph_hip_smooth_gauges_noalloc", "rrsph_hip_set_piston", "rrsph_hip_adjoint_forward", "rrsph_hip_adjoint_backward",
     "rrsph_hip_probe",
     "RRSPH_HIP_GetSpecificVersionMajor",
@@ -164,6 +164,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "rrsph_hip_kernel_class_name": (ctypes.c_char_p, [ctypes.c_uint32]),
         "rrsph_hip_time_steps": (st, [vp, ctypes.c_uint32, dp]),
         "rrsph_hip_sort_stats": (st, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64), up]),
+        "rrsph_hip_list_stats": (st, [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                       ctypes.POINTER(ctypes.c_uint64), up]),
         "rrsph_hip_smooth_gauges": (st, [vp, ctypes.POINTER(GaugeSpec), dp]),
         "rrsph_hip_smooth_gauges_noalloc": (st, [vp, ctypes.POINTER(GaugeSpec), dp]),
@@ -379,6 +381,15 @@ class HipStep:
         self._check(self.L.rrsph_hip_sort_stats(self.ctx, ctypes.byref(inc), ctypes.byref(full), ctypes.byref(fb),
                                                 ctypes.byref(last)))
         return {"incremental": inc.value, "full": full.value, "fallbacks": fb.value, "last_changed": last.value}
+
+    def list_stats(self) -> Dict[str, int]:
+        """Steps that read the neighbour lists, build steps, halts on
+        possibly stale lists, and the current build interval (0: no lists)."""
+        ls, b, h = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        iv = ctypes.c_uint32()
+        self._check(self.L.rrsph_hip_list_stats(self.ctx, ctypes.byref(ls), ctypes.byref(b), ctypes.byref(h),
+                                                ctypes.byref(iv)))
+        return {"list_steps": ls.value, "builds": b.value, "stale_halts": h.value, "interval": iv.value}
 
     def profile_steps(self, n: int):
         k = self.L.rrsph_hip_kernel_class_count()

@@ -83,6 +83,7 @@ def test_incremental_resort_above_2_to_the_20_equals_full_sort(monkeypatch):
     prm = lay.params
     assert prm.ntotal > 1 << 20
     print("re-sort above 2^20: n", prm.ntotal, flush=True)
+    monkeypatch.setenv("RRSPH_HIP_LIST", "0")  # the sorted step of every step
     monkeypatch.setenv("RRSPH_HIP_SORT", "full")
     warm = H.HipStep(prm, precision=H.FP32)
     monkeypatch.delenv("RRSPH_HIP_SORT")

@@ -271,6 +271,7 @@ def test_incremental_resort_equals_full_sort(monkeypatch, precision):
     lay = G.dam_break()
     prm = lay.params
     print("incremental re-sort, dam break n", prm.ntotal, "precision", precision, flush=True)
+    monkeypatch.setenv("RRSPH_HIP_LIST", "0")  # the sorted step of every step
     full, sf = _run_with_env(monkeypatch, {"RRSPH_HIP_SORT": "full"}, prm, lay.state, 200, precision)
     inc, si = _run_with_env(monkeypatch, {}, prm, lay.state, 200, precision)
     assert sf["full"] == 200 and sf["incremental"] == 0
@@ -291,6 +292,7 @@ def test_unstaged_resort_placement_equals_full_sort(monkeypatch):
     dam-break steps stay bit-identical to the full radix sort."""
     lay = G.dam_break()
     prm = lay.params
+    monkeypatch.setenv("RRSPH_HIP_LIST", "0")
     full, _ = _run_with_env(monkeypatch, {"RRSPH_HIP_SORT": "full"}, prm, lay.state, 200, H.FP32)
     inc, si = _run_with_env(monkeypatch, {"RRSPH_HIP_RESORT_STAGE": "0"}, prm, lay.state, 200, H.FP32)
     assert si["incremental"] == 199 and si["fallbacks"] == 0
