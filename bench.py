@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--adjoint3d-every", type=int, default=50)
     ap.add_argument("--adjoint3d-delta", type=float, default=0.01)
     ap.add_argument("--adjoint3d-fd", type=int, default=8, help="coefficients checked by central differences")
-    ap.add_argument("--steps-3d", type=int, default=20, help="steps of the 3D dam break (0: skip)")
+    ap.add_argument("--steps-3d", type=int, default=60, help="steps of the 3D dam break (0: skip)")
     ap.add_argument("--delta-3d", type=float, default=0.0025, help="spacing of the 3D dam break")
     return ap.parse_args()
 
@@ -212,9 +212,10 @@ def adjoint3d_leg(steps, every, delta, nfd):
     return out
 
 
-def leg_3d(steps, delta):
+def leg_3d(steps, delta, renorm_every=30):
     """3D dam break (y up): box 1.6 x 0.8 x 0.6 m, fluid block 0.4 x 0.3 x 0.6 m,
-    h = 1.3 delta, summation density, 3-layer LJ walls; FP32 and FP64."""
+    h = 1.3 delta, summation density renormalised every 30th step, 3-layer LJ
+    walls; FP32 and FP64."""
     import torch
 
     from sph2d_amd import generators as G
@@ -222,9 +223,11 @@ def leg_3d(steps, delta):
 
     lay = G.dam_break_3d(delta=delta, box=(1.6, 0.8, 0.6), column=(0.4, 0.3, 0.6), jitter=0.0)
     prm = lay.params
+    prm.density_normalization = 1
+    prm.density_normalization_every = renorm_every
     out = {"workload": "3D dam break 1.6 x 0.8 x 0.6 m box, 0.4 x 0.3 x 0.6 m fluid block, delta=%g m, h=1.3 delta, "
-                       "%d particles (%d fluid), summation density, cubic kernels, fixed dt" % (delta, prm.ntotal,
-                                                                                              prm.nfluid),
+                       "%d particles (%d fluid), summation density renormalised every %dth step, cubic kernels, "
+                       "fixed dt" % (delta, prm.ntotal, prm.nfluid, renorm_every),
            "particles": prm.ntotal}
     states = {}
     for name, prec in (("fp32", H.FP32), ("fp64", H.FP64)):

@@ -118,6 +118,17 @@ typedef struct rrsph_params_c {
      * (segments * nharm <= RRSPH_MAX_HARMONICS).  0 or 1: one piston. */
     int32_t piston_segments;
     double piston_seg_zmin, piston_seg_zmax;
+
+    /* Boundary-deficiency correction of the summation density
+     * (density_normalization = 1, DENSITY_NORMALIZATION_BASE of the
+     * reference's ParamsEnumeration.h:9-10, documented in
+     * docs/latex/ModelParams.tex:124-133, rejected by its loader): on every
+     * density_normalization_every-th step (1: every step) the summation
+     * density becomes rho_j = sum_i m W_ji / sum_i (m / rho_i) W_ji, the sums
+     * over j and its neighbours, rho_i the density before the step.  Only
+     * with density_treatment 0. */
+    int32_t density_normalization;
+    uint32_t density_normalization_every;
 } rrsph_params_c;
 
 typedef struct rrsph_ctx rrsph_ctx;

@@ -51,6 +51,8 @@ struct Options {
     // velocity sum_k a_k cos(k w t) + b_k sin(k w t), at most 32 harmonics
     double piston_omega = 0;
     std::vector<double> piston_a, piston_b;
+    // density_normalization (summation density): renormalise every this many steps
+    uint32_t density_normalization_every = 1;
 };
 
 // c^2 and c of the EOS as the OpenMP backend computes them (eos_sound_vel_method 1
@@ -88,6 +90,8 @@ template <class Params> inline rrsph_params_c make_params(const Params& p, const
     c.density_treatment = (int32_t)p.density_treatment;
     c.density_skf = (int32_t)p.density_skf;
     c.density_delta_sph_coef = p.density_delta_sph_coef;
+    c.density_normalization = (int32_t)p.density_normalization;
+    c.density_normalization_every = o.density_normalization_every;
     c.intf_sph_approximation = (int32_t)p.intf_sph_approximation;
     c.intf_skf = (int32_t)p.intf_skf;
     c.artificial_pressure = p.artificial_pressure ? 1 : 0;

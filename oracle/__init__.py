@@ -56,6 +56,7 @@ class OracleParams(ctypes.Structure):
         ("piston_seg_zmax", ctypes.c_double),
         ("boundary_motion", ctypes.c_int32), ("dt_dynamic_mode", ctypes.c_int32),
         ("num_threads", ctypes.c_int32),
+        ("density_normalization", ctypes.c_int32), ("density_normalization_every", ctypes.c_uint32),
     ]
 
 
@@ -91,6 +92,8 @@ def lib() -> ctypes.CDLL:
             "oracle_probe": (ctypes.c_int32, [P, c_double_p, c_int32_p, c_double_p, ctypes.c_uint32, ctypes.c_uint32,
                                               c_double_p, ctypes.c_uint32, c_double_p]),
             "oracle_density_sum": (None, [P, c_double_p, c_uint32_p, c_double_p, c_double_p]),
+            "oracle_density_sum_normalized": (None, [P, c_double_p, c_uint32_p, c_double_p, c_double_p,
+                                                     c_double_p]),
             "oracle_density_con": (None, [P, c_double_p, c_double_p, c_uint32_p, c_double_p, c_double_p,
                                           c_double_p]),
             "oracle_term_external": (None, [P, c_double_p, c_int32_p, c_uint32_p, c_double_p]),
@@ -134,7 +137,7 @@ def from_params(p, num_threads: int = 0) -> OracleParams:
     o = OracleParams()
     for name, _t in OracleParams._fields_:
         if name in ("piston_a", "piston_b", "piston_nharm", "num_threads", "eos_c_sqr", "eos_c", "piston_segments",
-                    "piston_seg_zmin", "piston_seg_zmax"):
+                    "piston_seg_zmin", "piston_seg_zmax", "density_normalization_every"):
             continue
         val = getattr(p, name)
         setattr(o, name, type(getattr(o, name))(val))
@@ -151,6 +154,7 @@ def from_params(p, num_threads: int = 0) -> OracleParams:
         o.piston_a[k] = a
         o.piston_b[k] = b
     o.num_threads = num_threads
+    o.density_normalization_every = max(int(getattr(p, "density_normalization_every", 1)), 1)
     return o
 
 

@@ -321,17 +321,26 @@ inline void for_pairs(const oracle_params& P, const uint32_t* nbr, uint32_t j, F
 
 // Density.h:25-61
 template <int D>
-void density_sum_t(const oracle_params& P, const double* r, const uint32_t* nbr, double* rho, double* pres) {
+// With rho_old (boundary-deficiency correction, docs/latex/ModelParams.tex:
+// 124-133): rho_j = sum_i m W_ji / sum_i (m / rho_old_i) W_ji, both sums over
+// j itself and its neighbours.  rho may not alias rho_old.
+void density_sum_t(const oracle_params& P, const double* r, const uint32_t* nbr, double* rho, double* pres,
+                   const double* rho_old = nullptr) {
     Kern K{P};
 #pragma omp parallel for schedule(static)
     for (int64_t jj = 0; jj < (int64_t)P.ntotal; ++jj) {
         const uint32_t j = (uint32_t)jj;
-        double s = P.mass * K.w(0, P.density_skf);
+        const double w0 = K.w(0, P.density_skf);
+        double s = P.mass * w0;
+        double sw = rho_old ? P.mass / rho_old[j] * w0 : 0.0;
         V<D> rj = V<D>::load(r, j);
         for_pairs<D>(P, nbr, j, [&](uint32_t i) {
             V<D> d = V<D>::load(r, i) - rj;
-            s += P.mass * K.w(len(d), P.density_skf);
+            const double w = K.w(len(d), P.density_skf);
+            s += P.mass * w;
+            if (rho_old) sw += P.mass / rho_old[i] * w;
         });
+        if (rho_old) s = s / sw;
         rho[j] = s;
         pres[j] = eos_p(P, s);
     }
@@ -526,7 +535,7 @@ double solitary_velocity(const oracle_params& P, double time) {
 // ==================================================================== context
 struct oracle_ctx {
     oracle_params P;
-    std::vector<double> r, v, rho, p, drho, a, av, rho_pred, v_pred, arvmu;
+    std::vector<double> r, v, rho, p, drho, a, av, rho_pred, v_pred, arvmu, rho_old;
     std::vector<int32_t> itype;
     std::vector<uint32_t> nbr, grid, start;
     std::vector<uint64_t> key;
@@ -553,10 +562,17 @@ static int32_t run_steps(oracle_ctx* c, uint32_t nsteps) {
         int32_t st = grid_find_t<D>(P, c->r.data(), c->itype.data(), c->nbr.data(), c->grid, c->start, c->key);
         if (st) return st;
         // density, TimeIntegration.cpp:74-83
-        if (cont)
+        // the renormalised summation on every density_normalization_every-th step
+        const uint32_t every = P.density_normalization_every ? P.density_normalization_every : 1;
+        const bool renorm = !cont && P.density_normalization && (c->step + 1) % every == 0;
+        if (cont) {
             density_con_t<D>(P, c->r.data(), c->v.data(), c->nbr.data(), c->rho.data(), c->drho.data(), c->p.data());
-        else
+        } else if (renorm) {
+            c->rho_old = c->rho;
+            density_sum_t<D>(P, c->r.data(), c->nbr.data(), c->rho.data(), c->p.data(), c->rho_old.data());
+        } else {
             density_sum_t<D>(P, c->r.data(), c->nbr.data(), c->rho.data(), c->p.data());
+        }
         // update_acceleration + update_dt, UpdateAcceleration.cpp:15-124
         double amax = 0, mumax = 0;
         update_acceleration_t<D>(P, c->itype.data(), c->r.data(), c->v_pred.data(), c->nbr.data(),
@@ -737,6 +753,13 @@ void oracle_density_sum(const oracle_params* p, const double* r, const uint32_t*
     set_threads(p);
     if (p->dim == 3) density_sum_t<3>(*p, r, nb, rho, pres);
     else density_sum_t<2>(*p, r, nb, rho, pres);
+}
+
+void oracle_density_sum_normalized(const oracle_params* p, const double* r, const uint32_t* nb, const double* rho_old,
+                                   double* rho, double* pres) {
+    set_threads(p);
+    if (p->dim == 3) density_sum_t<3>(*p, r, nb, rho, pres, rho_old);
+    else density_sum_t<2>(*p, r, nb, rho, pres, rho_old);
 }
 
 void oracle_density_con(const oracle_params* p, const double* r, const double* v, const uint32_t* nb,

@@ -104,6 +104,10 @@ typedef struct oracle_params {
     int32_t boundary_motion;
     int32_t dt_dynamic_mode;
     int32_t num_threads;         /* 0: OpenMP default */
+    /* summation density renormalised (ModelParams.tex:124-133) on every
+     * density_normalization_every-th step */
+    int32_t density_normalization;
+    uint32_t density_normalization_every;
 } oracle_params;
 
 typedef struct oracle_ctx oracle_ctx;
@@ -137,6 +141,10 @@ int32_t oracle_grid_find(const oracle_params* p, const double* r, const int32_t*
                          uint32_t* neighbours);
 void oracle_density_sum(const oracle_params* p, const double* r, const uint32_t* neighbours,
                         double* rho, double* pres);
+/* renormalised summation density: rho = sum m W / sum (m / rho_old) W
+ * (docs/latex/ModelParams.tex:124-133), self term included in both sums */
+void oracle_density_sum_normalized(const oracle_params* p, const double* r, const uint32_t* neighbours,
+                                   const double* rho_old, double* rho, double* pres);
 void oracle_density_con(const oracle_params* p, const double* r, const double* v,
                         const uint32_t* neighbours, const double* rho, double* drho, double* pres);
 /* per-term sums over the neighbour list, as the reference's module tests do */

@@ -70,6 +70,8 @@ struct Ctrl {
     double disp_last;       // the largest displacement of the last committed list or build step
     uint32_t list_overflow; // the last build found more candidates for some particle than its list holds
     uint32_t list_valid;    // the lists describe the current slot order (a build ran; no sort since)
+    uint32_t cur_renorm;    // this step's summation density is renormalised
+    uint32_t renorm_pad;
 };
 
 constexpr int kMaxSegments = 8;  // paddles of a segmented piston
@@ -92,6 +94,8 @@ struct StepParams {
     int32_t skf_density, skf_intf, skf_av, skf_xsph, skf_ap;
     double eos_B;
     int32_t density_treatment;
+    int32_t renorm;         // summation density renormalised on every renorm_every-th step
+    uint32_t renorm_every;
     double dhc;             // delta-SPH: coef * h * c
     int32_t intf_approx;
     int32_t art_pressure;

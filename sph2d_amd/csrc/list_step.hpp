@@ -254,15 +254,20 @@ __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* _
     const KernelConsts<R> K = kernel_consts<R>(P);
     const R m = (R)P.mass;
     const real4<R> vaj = X.va[s];
-    R rho = m * kernel_w<R>(K, P.skf_density, R(0));
+    const R w0 = kernel_w<R>(K, P.skf_density, R(0));
+    R rho = m * w0;
+    const bool renorm = ctrl->cur_renorm != 0;  // ModelParams.tex:124-133, rho_i before the step
+    R wsum = renorm ? m / vaj.z * w0 : R(0);
     bool overflow = false;
     if ((int)vaj.w != 0) {
         for_each_listed<R>(P, P.max_nbr - 1, s, X.pos[s], X.pos, X.va, pred, nbl, nbc[s], overflow,
-                           [&](bool take, double dx, double dy, const real4<R>&, const real4<R>&) {
-                               const R w = kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0));
-                               rho += take ? m * w : R(0);
+                           [&](bool take, double dx, double dy, const real4<R>& vai, const real4<R>&) {
+                               const R w = take ? kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0)) : R(0);
+                               rho += m * w;
+                               if (renorm) wsum += m / vai.z * w;
                            });
     }
+    if (renorm) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     real4<R> pr = pred[s];
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);

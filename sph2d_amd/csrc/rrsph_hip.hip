@@ -688,6 +688,10 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     P.skf_ap = p.artificial_pressure_skf;
     P.eos_B = p.eos_c_sqr * p.rho0 / 7;
     P.density_treatment = p.density_treatment;
+    if (p.density_normalization && p.density_treatment != 0)
+        return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: density_normalization needs density_treatment 0");
+    P.renorm = p.density_normalization ? 1 : 0;
+    P.renorm_every = p.density_normalization_every ? p.density_normalization_every : 1u;
     P.dhc = p.density_delta_sph_coef * p.hsml * p.eos_c;
     P.intf_approx = p.intf_sph_approximation;
     P.art_pressure = p.artificial_pressure;

@@ -224,17 +224,25 @@ __global__ void __launch_bounds__(kBlock) density_sum3_kernel(StepParams P, Ctrl
     if (s >= P.n) return;
     const R m = (R)P.mass, inv_h = (R)(1.0 / P.h), w_q1 = (R)P.f_cubic, w_q2 = (R)(P.f_cubic / 6.0);
     const double4 pj = B.pos[s];
-    R rho = m * w_q1 * (R(2) / R(3));
+    const R w0 = w_q1 * (R(2) / R(3));
+    R rho = m * w0;
+    // renormalised step: the denominator sum_i (m / rho_i) W_ji, rho_i the
+    // density before the step (ModelParams.tex:124-133)
+    const bool renorm = ctrl->cur_renorm != 0;
+    R wsum = renorm ? m / B.va[s].w * w0 : R(0);
     bool overflow = false;
     if ((int)pj.w != 0) {
         for_each_neighbour3(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
-                            [&](uint32_t, double dx, double dy, double dz) {
+                            [&](uint32_t i, double dx, double dy, double dz) {
                                 const R q = pair_dist<R>(dx, dy, dz) * inv_h;
                                 const R t = R(2) - q;
-                                rho += m * (q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q)
-                                                      : w_q2 * t * t * t);
+                                const R w = q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q)
+                                                      : w_q2 * t * t * t;
+                                rho += m * w;
+                                if (renorm) wsum += m / B.va[i].w * w;
                             });
     }
+    if (renorm) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     real4<R> pr = B.pred[s];
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);

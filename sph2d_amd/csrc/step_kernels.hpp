@@ -185,6 +185,9 @@ __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ct
     c.cur_time = c.time;
     c.cur_dt = c.dt;
     c.cur_first = c.step == 0;
+    // the boundary-deficiency correction of the summation density on every
+    // renorm_every-th step (steps counted from 1)
+    c.cur_renorm = P.renorm && (c.step + 1) % P.renorm_every == 0;
     // dt of whole_step when it does not depend on this step's reductions
     if (P.dt_method == 2 && P.dt_dynamic_mode == 0) {
         // RRSPH_OMP OpenMP build: max |a| and max mu stay 0 (UpdateAcceleration.cpp:107-124)
@@ -499,15 +502,23 @@ __global__ void __launch_bounds__(kBlock) density_sum_kernel(StepParams P, Ctrl*
     const KernelConsts<R> K = kernel_consts<R>(P);
     const R m = (R)P.mass;
     const real4<R> va = B.va[s];
-    R rho = m * kernel_w<R>(K, P.skf_density, R(0));
+    const R w0 = kernel_w<R>(K, P.skf_density, R(0));
+    R rho = m * w0;
+    // renormalised step: the denominator sum_i (m / rho_i) W_ji, rho_i the
+    // density before the step (ModelParams.tex:124-133)
+    const bool renorm = ctrl->cur_renorm != 0;
+    R wsum = renorm ? m / va.z * w0 : R(0);
     bool overflow = false;
     if ((int)va.w != 0) {
         const double2 pj = B.pos[s];
         for_each_neighbour<false>(P, P.max_nbr - 1, s, skey[s], pj, B.pos, LB, overflow,
-                                  [&](uint32_t, double dx, double dy) {
-                                      rho += m * kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0));
+                                  [&](uint32_t i, double dx, double dy) {
+                                      const R w = kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0));
+                                      rho += m * w;
+                                      if (renorm) wsum += m / B.va[i].z * w;
                                   });
     }
+    if (renorm) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     real4<R> pr = B.pred[s];
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);

@@ -79,6 +79,7 @@ class RrsphParams(ctypes.Structure):
         ("precision", ctypes.c_int32), ("start_time", ctypes.c_double),
         ("piston_segments", ctypes.c_int32), ("piston_seg_zmin", ctypes.c_double),
         ("piston_seg_zmax", ctypes.c_double),
+        ("density_normalization", ctypes.c_int32), ("density_normalization_every", ctypes.c_uint32),
     ]
 
 
@@ -189,7 +190,7 @@ def make_c_params(prm: "P.ExperimentParams", precision: int = FP64) -> RrsphPara
     c = RrsphParams()
     c.struct_size = ctypes.sizeof(RrsphParams)
     skip = {"struct_size", "eos_c_sqr", "eos_c", "piston_a", "piston_b", "piston_nharm", "precision", "start_time",
-            "piston_segments", "piston_seg_zmin", "piston_seg_zmax"}
+            "piston_segments", "piston_seg_zmin", "piston_seg_zmax", "density_normalization_every"}
     for name, _t in RrsphParams._fields_:
         if name in skip:
             continue
@@ -210,6 +211,7 @@ def make_c_params(prm: "P.ExperimentParams", precision: int = FP64) -> RrsphPara
         c.piston_b[k] = b
     c.precision = int(precision)
     c.start_time = prm.start_simulation_time
+    c.density_normalization_every = max(int(getattr(prm, "density_normalization_every", 1)), 1)
     return c
 
 

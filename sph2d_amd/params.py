@@ -117,7 +117,8 @@ _MODEL_KEYS = [
 # Keys of the RRSPH_HIP extension (Fourier-series piston, optionally split
 # into paddles across z); they are ignored by the reference loader and
 # optional here.
-_EXTENSION_KEYS = ("piston_omega", "piston_a", "piston_b", "piston_segments", "piston_seg_zmin", "piston_seg_zmax")
+_EXTENSION_KEYS = ("piston_omega", "piston_a", "piston_b", "piston_segments", "piston_seg_zmin", "piston_seg_zmax",
+                   "density_normalization_every")
 
 
 def _version_tuple(d: Dict[str, Any]):
@@ -142,8 +143,11 @@ def check_optional_params(mp: Dict[str, Any]) -> None:
 
     if mp["visc"]:
         raise RuntimeError("Not implemented: visc")
-    if mp["density_normalization"]:
-        raise RuntimeError("Not implemented: density_normalization")
+    # the reference's loader rejects density_normalization as not implemented;
+    # this backend implements it as its documentation states it
+    # (docs/latex/ModelParams.tex:124-133: summation density only)
+    if mp["density_normalization"] and mp["density_treatment"] != DENSITY_SUMMATION:
+        raise RuntimeError("density_normalization needs density_treatment == 0")
     if mp["eos_sound_vel_method"] == EOS_SOUND_VEL_SPECIFIC:
         need("eos_sound_vel")
     elif mp["eos_sound_vel_method"] == EOS_SOUND_VEL_DAM_BREAK:
@@ -297,6 +301,8 @@ class ExperimentParams:
     delta: float = 0.0
     density_treatment: int = DENSITY_CONTINUITY
     density_normalization: int = 0
+    # extension: the boundary-deficiency correction on every n-th step only
+    density_normalization_every: int = 1
     density_delta_sph_coef: float = 0.1
     average_velocity: bool = True
     average_velocity_coef: float = 0.3
@@ -363,6 +369,8 @@ def apply_model_params(params: ExperimentParams, mp: Dict[str, Any]) -> None:
         params.piston_omega = float(mp["piston_omega"])
         params.piston_a = tuple(float(x) for x in mp["piston_a"])
         params.piston_b = tuple(float(x) for x in mp["piston_b"])
+    if mp.get("density_normalization_every") is not None:
+        params.density_normalization_every = int(mp["density_normalization_every"])
     if mp.get("piston_segments") is not None:
         params.piston_segments = int(mp["piston_segments"])
         params.piston_seg_zmin = float(mp["piston_seg_zmin"])

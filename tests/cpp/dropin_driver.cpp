@@ -32,7 +32,7 @@ struct Params {
            pi = 3.14159265358979323846;
     unsigned eos_sound_vel_method = 0;
     double eos_sound_vel_coef = 1, eos_sound_vel = 0;
-    unsigned density_treatment = 1, density_skf = 1;
+    unsigned density_treatment = 1, density_skf = 1, density_normalization = 0;
     double density_delta_sph_coef = 0.1;
     unsigned intf_sph_approximation = 2, intf_skf = 1;
     bool artificial_pressure = false;

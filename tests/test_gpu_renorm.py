@@ -1,0 +1,62 @@
+"""The renormalised summation density (the boundary-deficiency correction of
+docs/latex/ModelParams.tex:124-133, density_normalization = 1) on the GPU
+against the FP64 oracle: rho_j = sum m W / sum (m / rho_i) W on every n-th
+step, plain summation in between; 2D (list and sorted steps) and 3D."""
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+from sph2d_amd import generators as G
+from sph2d_amd import hip as H
+from sph2d_amd import params as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(lay, prm, nsteps, precision):
+    gpu = H.HipStep(prm, precision=precision)
+    gpu.upload(lay.r, lay.v, lay.rho, lay.itype)
+    gpu.step(nsteps)
+    g = gpu.download()
+    gpu.close()
+    ora = oracle.Oracle(prm)
+    ora.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=prm.dt)
+    assert ora.step(nsteps) == 0
+    return g, ora.state()
+
+
+@pytest.mark.parametrize("lists", ["1", "0"])
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_renormalised_summation_2d(monkeypatch, lists, precision):
+    """C0 dam break, summation density renormalised every 3rd step, 9 steps."""
+    monkeypatch.setenv("RRSPH_HIP_LIST", lists)
+    lay = G.dam_break(density_treatment=P.DENSITY_SUMMATION)
+    prm = copy.copy(lay.params)
+    prm.density_normalization = 1
+    prm.density_normalization_every = 3
+    g, r = _compare(lay, prm, 9, precision)
+    plain_prm = copy.copy(lay.params)
+    gp, _ = _compare(lay, plain_prm, 9, H.FP64) if precision == H.FP64 else (None, None)
+    rho_err = float(np.max(np.abs(g["rho"] - r["rho"]) / r["rho"]))
+    if precision == H.FP64:
+        assert rho_err < 1e-12 and np.max(np.abs(g["r"] - r["r"])) < 1e-13
+        assert np.max(np.abs(gp["rho"] - r["rho"])) > 1.0  # the correction changed the run
+    else:
+        assert rho_err < 1e-5 and np.max(np.abs(g["r"] - r["r"])) < 1e-6 * prm.delta
+
+
+@pytest.mark.parametrize("precision", [H.FP64, H.FP32])
+def test_renormalised_summation_3d(precision):
+    """The small 3D dam break (delta 0.04 m), renormalised every 2nd step, 6 steps."""
+    lay = G.dam_break_3d(delta=0.04)
+    prm = copy.copy(lay.params)
+    prm.density_normalization = 1
+    prm.density_normalization_every = 2
+    g, r = _compare(lay, prm, 6, precision)
+    rho_err = float(np.max(np.abs(g["rho"] - r["rho"]) / r["rho"]))
+    if precision == H.FP64:
+        assert rho_err < 1e-12 and np.max(np.abs(g["r"] - r["r"])) < 1e-13
+    else:
+        assert rho_err < 1e-5

@@ -16,6 +16,10 @@ and with closed forms the mathematics fixes.
   order: Dean & Dalrymple's piston transfer function; second order: Madsen
   1971; solitary: Rayleigh's paddle displacement as used by Dominguez 2019),
   and the wall-disappear maker kills exactly its particle range, once.
+* The renormalised summation density (docs/latex/ModelParams.tex:124-133,
+  implemented here though the reference's loader rejects it): a uniform
+  old density is reproduced exactly at any positions, the two-particle
+  closed form, and the every-n-th-step schedule in the time loop.
 
 Every table value is checked with the reference test's own inputs and its
 checkDouble<N> (testSPH.h:51-97), and again at the precision it was printed
@@ -599,3 +603,82 @@ def test_wall_disappear_kills_exactly_its_range_once():
     assert np.array_equal(it[~dead], lay.itype[~dead])
     assert ora.step(5) == 0
     assert np.array_equal(ora.state()["itype"], it)
+
+
+# ---------------------------------------------------------------- renormalised summation
+def _renorm(prm, r, itype, rho_old):
+    st, nb = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    op = oracle.from_params(prm)
+    n = prm.ntotal
+    rho, p = np.zeros(n), np.zeros(n)
+    oracle.lib().oracle_density_sum_normalized(ctypes.byref(op), oracle.ptr(np.ascontiguousarray(r)),
+                                               oracle.ptr(np.ascontiguousarray(nb.reshape(-1)), ctypes.c_uint32),
+                                               oracle.ptr(np.ascontiguousarray(rho_old)), oracle.ptr(rho),
+                                               oracle.ptr(p))
+    plain, pp = np.zeros(n), np.zeros(n)
+    oracle.lib().oracle_density_sum(ctypes.byref(op), oracle.ptr(np.ascontiguousarray(r)),
+                                    oracle.ptr(np.ascontiguousarray(nb.reshape(-1)), ctypes.c_uint32),
+                                    oracle.ptr(plain), oracle.ptr(pp))
+    return rho, plain
+
+
+def test_renormalised_density_reproduces_a_uniform_density():
+    """docs/latex/ModelParams.tex:124-133: rho_j = sum m W / sum (m / rho_i) W.
+    With every rho_i = c the ratio is c, whatever the positions: at the free
+    surface and the walls of the jittered dam break too, where the plain
+    summation falls short of rho0 (the boundary deficiency it corrects)."""
+    lay = G.dam_break()
+    prm = copy.copy(lay.params)
+    prm.density_treatment = P.DENSITY_SUMMATION
+    for c in (1000.0, 1020.5):
+        rho, plain = _renorm(prm, lay.r, lay.itype, np.full(prm.ntotal, c))
+        assert np.max(np.abs(rho - c)) <= 4e-16 * c * 8
+    assert plain.min() < 0.8 * prm.rho0  # free surface and corners: missing kernel support
+
+
+def test_renormalised_density_two_particle_closed_form():
+    """Two particles r apart: rho_j = (m W(0) + m W(r)) / ((m/rho_j) W(0) + (m/rho_i) W(r))."""
+    lay = G.dam_break()
+    prm = copy.copy(lay.params)
+    prm.density_treatment = P.DENSITY_SUMMATION
+    prm.ntotal = prm.nfluid = 2
+    h = prm.hsml
+    for q in (0.4, 1.3, 1.9):
+        r = np.array([[1.0, 0.5], [1.0 + q * h * 0.6, 0.5 + q * h * 0.8]])
+        rho_old = np.array([1003.0, 991.0])
+        rho, plain = _renorm(prm, r, np.array([2, 2], dtype=np.int32), rho_old)
+        op = oracle.from_params(prm)
+        w0 = oracle.lib().oracle_kernel_w(ctypes.byref(op), 0.0, 1)
+        w = oracle.lib().oracle_kernel_w(ctypes.byref(op), q * h, 1)
+        m = prm.mass
+        for j, i in ((0, 1), (1, 0)):
+            want = (m * w0 + m * w) / (m / rho_old[j] * w0 + m / rho_old[i] * w)
+            assert rho[j] == pytest.approx(want, rel=1e-14)
+            assert plain[j] == pytest.approx(m * (w0 + w), rel=1e-14)
+
+
+def test_renormalisation_every_third_step_in_the_time_loop():
+    """density_normalization_every = 3: steps 1 and 2 are the plain
+    summation, step 3 renormalises with the densities after step 2."""
+    lay = G.dam_break(density_treatment=P.DENSITY_SUMMATION)
+    prm = copy.copy(lay.params)
+    plain = oracle.Oracle(prm)
+    plain.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=prm.dt)
+    prm_n = copy.copy(prm)
+    prm_n.density_normalization = 1
+    prm_n.density_normalization_every = 3
+    norm = oracle.Oracle(prm_n)
+    norm.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=prm.dt)
+    assert plain.step(2) == 0 and norm.step(2) == 0
+    a, b = plain.state(), norm.state()
+    for k in ("r", "v", "rho", "p"):
+        assert np.array_equal(a[k], b[k]), k
+    assert norm.step(1) == 0 and plain.step(1) == 0
+    b3 = norm.fields()
+    s3 = norm.state()
+    # step 3's density is the renormalised summation of the state before it
+    rho3, _ = _renorm(prm_n, b["r"], b["itype"], b["rho"])
+    assert np.max(np.abs(s3["rho"] - rho3) / rho3) < 1e-14
+    assert np.max(np.abs(s3["rho"] - plain.state()["rho"])) > 1.0  # and it differs from the plain step
+    assert np.all(np.isfinite(b3["a"]))
