@@ -6,21 +6,37 @@
 // every particle i with |r_i - r_j| < k h + skin goes into j's list, in the
 // walk order, k-major (entry k of slot j at nbl[k n + j], count nbc[j]).
 // Until the next build the slot order stays as it is, and a step reads the
-// lists instead of walking the cells: the pair pass applies the reference's
-// exact FP64 cut-off test (GridFind.cpp:144-176, |r|^2 < (k h)^2 with
-// separate roundings) to each entry, so the neighbour sets are those of the
-// cell walk.  A pair closer than k h now was closer than k h + skin at the
-// build as long as no particle has moved skin / 2 since: the integrator
-// reduces each step's largest displacement, the prologue sums them, and a
-// step whose lists may be stale halts before it reads them (status
+// lists instead of walking the cells: the pair pass decides each entry with
+// the reference's FP64 cut-off test (GridFind.cpp:144-176, |r|^2 < (k h)^2
+// with separate roundings), so the neighbour sets are those of the cell
+// walk.  A pair closer than k h now was closer than k h + skin at the build
+// as long as no particle has moved skin / 2 since: the integrator reduces
+// each step's largest displacement, the prologue sums them, and a step
+// whose lists may be stale halts before it reads them (status
 // kStatusRebuild); the host re-runs it as a build step.  The host schedules
 // builds ahead from the measured displacement rate, so such halts are rare.
 //
+// What the pair pass gathers per entry, two records in slot order:
+//   near  FP32: (xi_x, xi_y, vx, vy), xi the position relative to the
+//         origin of the particle's home cell (its cell at the last build),
+//         rounded from the FP64 position; FP64: (x, y, vx, vy), the FP64
+//         position itself
+//   pred  (v^x, v^y, s rho^, p / rho^2): predicted velocity and density,
+//         s = +1 for fluid, -1 for boundary, 0 for a particle that no
+//         longer exists
+// An entry holds the neighbour's slot in its low 26 bits and the offset of
+// the neighbour's home cell from the particle's, (dcx + 3) | (dcy + 3) << 3,
+// in its high 6, so that in FP32 r_i - r_j = xi_i - xi_j + (dcx, dcy) cell
+// to a few 1e-9 m.  FP32 decides an entry from that difference unless its
+// squared distance lies within 1e-5 of the cut-off (or a position lies far
+// from its home cell), where it takes the FP64 positions and the reference's
+// test; the rounding of xi stays far below that band (2e-6 of the cut-off).
+//
 // Step kernels: list_prologue -> [density_sum_list] -> pair_list ->
-// integrate_list (in place; it also writes the next step's predicted
-// velocity and density, so no reorder runs between builds).  A build step
-// runs list_prologue -> radix sort -> reorder_kernel -> list_build_kernel ->
-// the same step kernels on the reordered copy, integrating into A.
+// integrate_list (in place; it also writes the next step's records, so no
+// reorder runs between builds).  A build step runs list_prologue -> radix
+// sort -> reorder_list_kernel -> list_build_kernel -> the same step kernels
+// on the reordered copy, integrating into A.
 #pragma once
 
 #include "device_common.hpp"
@@ -32,11 +48,88 @@ constexpr uint32_t kStatusRebuild = 101;       // internal: re-run this step as 
 constexpr uint32_t kStatusListOverflow = 102;  // internal: a list ran out of room; run without lists
 
 // The state a list step reads (the reordered copy B on a build step, A after).
+constexpr uint32_t kSlotBits = 26;
+constexpr uint32_t kSlotMask = (1u << kSlotBits) - 1;  // list steps take at most 2^26 particles
+
 template <class R> struct ListState {
     const double2* pos;
     const real4<R>* va;
     const uint32_t* oid;
 };
+
+// The sign-encoded predicted density of the pred record.
+template <class R> __device__ __forceinline__ R typed_rho(R rho, int itype) {
+    return itype > 0 ? rho : (itype < 0 ? -rho : R(0));
+}
+
+// The near record of a particle at pos with velocity (vx, vy) and home cell
+// `home` (a cell key cx ncy + cy).
+template <class R> __device__ __forceinline__ real4<R> near_record(const StepParams& P, double2 pos, R vx, R vy,
+                                                                   uint32_t home);
+template <> __device__ __forceinline__ float4 near_record<float>(const StepParams& P, double2 pos, float vx, float vy,
+                                                                 uint32_t home) {
+    const uint32_t cx = home / (uint32_t)P.ncy, cy = home - cx * (uint32_t)P.ncy;
+    const double ox = P.xmin + (double)cx * P.cell, oy = P.ymin + (double)cy * P.cell;
+    return make_float4((float)(pos.x - ox), (float)(pos.y - oy), vx, vy);
+}
+template <> __device__ __forceinline__ double4 near_record<double>(const StepParams&, double2 pos, double vx,
+                                                                   double vy, uint32_t) {
+    return make_double4(pos.x, pos.y, vx, vy);
+}
+
+// The prediction of PredictHalfStep.cpp:16-22 (v^ = v + a dt/2, rho^ = rho +
+// drho dt/2 with continuity) and the pressure of the current density
+// (Density.h:112) as reorder_kernel computes them; p / rho^2 in w.
+template <class R> __device__ __forceinline__ real4<R> predict_record(const StepParams& P, const real4<R>& va,
+                                                                      R ax, R ay, R drho, R dt, R& p) {
+    const bool cont = P.density_treatment != 0;
+    real4<R> pr;
+    pr.x = va.x + ax * dt * R(0.5);
+    pr.y = va.y + ay * dt * R(0.5);
+    pr.z = cont ? va.z + drho * dt * R(0.5) : va.z;
+    p = eos_p<R>(va.z, (R)P.rho0, (R)P.eos_B);
+    pr.w = p / (pr.z * pr.z);
+    pr.z = typed_rho<R>(pr.z, (int)va.w);
+    return pr;
+}
+
+// The reorder of a build step: reorder_kernel's gather, prediction,
+// pressure and cell-start table, plus the records of the list pass and each
+// slot's home cell (its cell key).
+template <class R>
+__global__ void __launch_bounds__(kBlock) reorder_list_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
+                                                              const uint32_t* __restrict__ skey,
+                                                              const uint32_t* __restrict__ perm, StateA<R> A,
+                                                              StateB<R> B, real4<R>* __restrict__ nearr,
+                                                              uint32_t* __restrict__ home, uint32_t* __restrict__ LB,
+                                                              unsigned long long* __restrict__ maxslots) {
+    if (ctrl->halted) return;
+    if (blockIdx.x == 0) {
+        maxslots[threadIdx.x] = 0;
+        maxslots[kMaxSlots + threadIdx.x] = 0;
+    }
+    const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = s0 < P.n;
+    const uint32_t s = live ? s0 : P.n - 1;
+    const uint32_t k = skey[s];
+    const uint32_t src = min(perm[s], P.n - 1);  // a permutation of [0, n); the clamp only guards memory
+    const double2 pos = A.pos[src];
+    const real4<R> va = A.va[src];
+    const real4<R> ad = A.ad[src];
+    const uint32_t oid = A.oid[src];
+    R p;
+    const real4<R> pr = predict_record<R>(P, va, ad.x, ad.y, ad.z, (R)ctrl->cur_dt, p);
+    if (live) {
+        B.pos[s] = pos;
+        B.va[s] = va;
+        B.oid[s] = oid;
+        B.pred[s] = pr;
+        B.pres[s] = p;
+        nearr[s] = near_record<R>(P, pos, va.x, va.y, k);
+        home[s] = k;
+    }
+    fill_cell_starts(P, s, live, k, skey, LB);
+}
 
 // Prologue of a list step or a build step (one block of kBlock threads):
 // the previous step's largest displacement (reduced from its slots), the
@@ -96,6 +189,7 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) list_build_kernel(StepParams P, Ctrl* __restrict__ ctrl,
                                                             const double2* __restrict__ pos,
                                                             const real4<R>* __restrict__ va,
+                                                            const uint32_t* __restrict__ skey,
                                                             const uint32_t* __restrict__ LB, uint32_t* __restrict__ nbl,
                                                             uint32_t* __restrict__ nbc) {
     if (ctrl->halted) return;
@@ -104,8 +198,11 @@ __global__ void __launch_bounds__(kBlock) list_build_kernel(StepParams P, Ctrl* 
     const uint32_t s = lb * kBlock + threadIdx.x;
     if (s >= P.n) return;
     uint32_t cnt = 0;
+    bool bad = false;
     if ((int)va[s].w != 0) {
         const double2 pj = pos[s];
+        const uint32_t kj = skey[s];
+        const int hx = (int)(kj / (uint32_t)P.ncy), hy = (int)kj - hx * P.ncy;
         const double reach = P.list_reach * (1.0 + 1e-9);
         const int x0 = clamped_cell(pj.x - reach, P.xmin, P.cell, P.ncx);
         const int x1 = clamped_cell(pj.x + reach, P.xmin, P.cell, P.ncx);
@@ -125,76 +222,128 @@ __global__ void __launch_bounds__(kBlock) list_build_kernel(StepParams P, Ctrl* 
                     const double dx = q[u].x - pj.x, dy = q[u].y - pj.y;
                     const bool in = (i + u < e) & (i + u != s) & (dx * dx + dy * dy < P.list_r2);
                     if (in) {
-                        if (cnt < P.list_cap) nbl[(size_t)cnt * P.n + s] = i + u;
+                        // the offset of the neighbour's home cell (the rows of a
+                        // column range are consecutive keys)
+                        const uint32_t ki = skey[i + u];
+                        const int dcx = col - hx, dcy = (int)ki - col * P.ncy - hy;
+                        bad |= dcx < -3 || dcx > 4 || dcy < -3 || dcy > 4;
+                        const uint32_t code = (uint32_t)(dcx + 3) | ((uint32_t)(dcy + 3) << 3);
+                        if (cnt < P.list_cap) nbl[(size_t)cnt * P.n + s] = (i + u) | (code << kSlotBits);
                         ++cnt;
                     }
                 }
             }
         }
     }
-    if (cnt > P.list_cap) atomicOr(&ctrl->list_overflow, 1u);
+    if (cnt > P.list_cap || bad) atomicOr(&ctrl->list_overflow, 1u);
     nbc[s] = min(cnt, P.list_cap);
 }
 
-// Visits the entries of slot s's list: f(take, dx, dy, va_i, pred_i) for
-// every entry, take = the entry passes the reference's exact test, is alive
-// and is among the first `cap` such (further ones set `overflow`,
-// GridFind.cpp:158-188); f must add nothing when take is false.  Every
-// entry runs the same instructions (most entries are neighbours, so
-// skipping the others would not shorten a wave's work).  The list index
-// two entries ahead and the positions, velocities and predictions one
-// entry ahead are in flight while f runs.
+// One list entry as the pair pass sees it.
+template <class R> struct ListEntry {
+    bool take;        // a neighbour: within the cut-off, alive, among the first `cap`
+    R dx, dy;         // r_i - r_j
+    real4<R> nr;      // near record of i: (xi or x, xi or y, vx, vy)
+    real4<R> pr;      // pred record of i: (v^x, v^y, s rho^, p / rho^2)
+};
+
+// r_i - r_j of an entry and whether it is within the cut-off.  FP32: from
+// the near records and the home-cell offset, the reference's FP64 test on
+// the FP64 positions only inside the band around the cut-off or when a
+// position lies more than four cells from its home cell's origin.  FP64:
+// the reference's test on the FP64 positions of the near records.
+struct CutFP32 {
+    float lo, hi, cellf, far;
+};
+__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32& C, uint32_t code, const float4& ni,
+                                         const float4& nj, uint32_t i, double2 pj, const double2* __restrict__ pos,
+                                         float& dx, float& dy) {
+    dx = (ni.x - nj.x) + (float)((int)(code & 7u) - 3) * C.cellf;
+    dy = (ni.y - nj.y) + (float)((int)(code >> 3) - 3) * C.cellf;
+    const float d2 = dx * dx + dy * dy;
+    const bool far = fmaxf(fabsf(ni.x), fabsf(ni.y)) > C.far || fmaxf(fabsf(nj.x), fabsf(nj.y)) > C.far;
+    if (!far && !(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
+    const double2 q = pos[i];  // rare: the band around the cut-off
+    const double ddx = __dsub_rn(q.x, pj.x), ddy = __dsub_rn(q.y, pj.y);
+    dx = (float)ddx;
+    dy = (float)ddy;
+    return dist2_exact(ddx, ddy) < P.cut2;
+}
+__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32&, uint32_t, const double4& ni,
+                                         const double4& nj, uint32_t, double2, const double2* __restrict__,
+                                         double& dx, double& dy) {
+    dx = __dsub_rn(ni.x, nj.x);
+    dy = __dsub_rn(ni.y, nj.y);
+    return dist2_exact(dx, dy) < P.cut2;
+}
+
+// Visits the entries of slot s's list: f(ListEntry) for every entry (f must
+// add nothing when take is false: every entry runs the same instructions,
+// most entries being neighbours).  Entries beyond the first `cap`
+// neighbours set `overflow` (GridFind.cpp:158-188).  Two register sets in
+// turn: while one entry is processed, the next one's records and the index
+// after it are in flight; each index is loaded before the gathers issued
+// with it.
 template <class R, class F>
-__device__ __forceinline__ void for_each_listed(const StepParams& P, uint32_t cap, uint32_t s, double2 pj,
-                                                const double2* __restrict__ pos, const real4<R>* __restrict__ va,
-                                                const real4<R>* __restrict__ pred, const uint32_t* __restrict__ nbl,
-                                                uint32_t cnt, bool& overflow, F&& f) {
+__device__ __forceinline__ void for_each_listed(const StepParams& P, uint32_t cap, uint32_t s,
+                                                const real4<R>& nj, double2 pj, const double2* __restrict__ pos,
+                                                const real4<R>* __restrict__ nearr, const real4<R>* __restrict__ pred,
+                                                const uint32_t* __restrict__ nbl, uint32_t cnt, bool& overflow,
+                                                F&& f) {
     if (cnt == 0) return;
     const size_t n = P.n;
     const uint32_t last = cnt - 1;
+    CutFP32 C;
+    C.lo = (float)(P.cut2 * (1.0 - 1e-5));
+    C.hi = (float)(P.cut2 * (1.0 + 1e-5));
+    C.cellf = (float)P.cell;
+    C.far = (float)(4.0 * P.cell);
     uint32_t hits = 0;
     bool over = false;
-    auto visit = [&](const double2& q, const real4<R>& a, const real4<R>& b) {
-        const double dx = __dsub_rn(q.x, pj.x), dy = __dsub_rn(q.y, pj.y);
-        const bool hit = (dist2_exact(dx, dy) < P.cut2) & ((int)a.w != 0);
-        const bool take = hit & (hits < cap);
-        over |= hit & !take;
-        hits += take ? 1u : 0u;
-        f(take, dx, dy, a, b);
+    auto visit = [&](uint32_t e, const real4<R>& ni, const real4<R>& pi) {
+        ListEntry<R> E;
+        const uint32_t i = e & kSlotMask;
+        const bool in = list_cut(P, C, e >> kSlotBits, ni, nj, i, pj, pos, E.dx, E.dy);
+        const bool hit = in & (pi.z != R(0));
+        E.take = hit & (hits < cap);
+        over |= hit & !E.take;
+        hits += E.take ? 1u : 0u;
+        E.nr = ni;
+        E.pr = pi;
+        f(E);
     };
-    // two register sets in turn (no copies between iterations): while one
-    // entry is processed the next one's gathers and the index after it are
-    // in flight; each index is loaded before the gathers issued with it
-    uint32_t iA = nbl[s];
-    uint32_t iB = nbl[min(1u, last) * n + s];
-    double2 qA = pos[iA], qB;
-    real4<R> aA = va[iA], bA = pred[iA], aB, bB;
+    uint32_t eA = nbl[s];
+    uint32_t eB = nbl[min(1u, last) * n + s];
+    real4<R> nA = nearr[eA & kSlotMask], pA = pred[eA & kSlotMask], nB, pB;
     uint32_t k = 0;
 #pragma unroll 1
     for (; k + 1 < cnt; k += 2) {
-        iA = nbl[min(k + 2, last) * n + s];
-        qB = pos[iB];
-        aB = va[iB];
-        bB = pred[iB];
-        visit(qA, aA, bA);
-        iB = nbl[min(k + 3, last) * n + s];
-        qA = pos[iA];
-        aA = va[iA];
-        bA = pred[iA];
-        visit(qB, aB, bB);
+        const uint32_t eA2 = nbl[min(k + 2, last) * n + s];
+        nB = nearr[eB & kSlotMask];
+        pB = pred[eB & kSlotMask];
+        visit(eA, nA, pA);
+        eA = eA2;
+        const uint32_t eB2 = nbl[min(k + 3, last) * n + s];
+        nA = nearr[eA & kSlotMask];
+        pA = pred[eA & kSlotMask];
+        visit(eB, nB, pB);
+        eB = eB2;
     }
-    if (k < cnt) visit(qA, aA, bA);
+    if (k < cnt) visit(eA, nA, pA);
     overflow |= over;
 }
 
-// fast_pair_term for one list entry without branches except the rare
-// Lennard-Jones case: both pieces of the cubic spline are evaluated and
-// selected, and nothing is added when take is false.  The Lennard-Jones
-// ratios use 1/|r| from the reciprocal square root instead of two divisions.
+// fast_pair_term (the cubic configuration) for one list entry, without
+// branches except the rare Lennard-Jones case: both pieces of the cubic
+// spline are evaluated and selected, and nothing is added when take is
+// false.  The Lennard-Jones ratios use 1/|r| from the reciprocal square
+// root instead of two divisions.  j's own records: vaj (v, rho, itype) and
+// prj (pred record).
 template <class R>
-__device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>& S, bool take, R dx, R dy,
-                                               const real4<R>& vai, const real4<R>& pri, const real4<R>& vaj,
-                                               const real4<R>& prj, int tj) {
+__device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>& S, const ListEntry<R>& E,
+                                               const real4<R>& vaj, const real4<R>& prj, int tj) {
+    const R dx = E.dx, dy = E.dy;
+    const bool take = E.take;
     const R d2 = dx * dx + dy * dy;
     const R rinv = d2 > R(0) ? fast_rsqrt<R>(d2) : R(0);
     const R dist = d2 * rinv;
@@ -208,18 +357,18 @@ __device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>
     const R W2 = K.w_q2 * t2 * t;
     const R F = take ? (inner ? F1 : F2) : R(0);
     const R W = inner ? W1 : W2;
-    if (K.cont) S.sd += F * ((vai.x - vaj.x) * dx + (vai.y - vaj.y) * dy);
-    const int ti = (int)vai.w;
-    if (K.lj_on && take && tj > 0 && ti < 0 && dist < K.lj_r0) {
+    if (K.cont) S.sd += F * ((E.nr.z - vaj.x) * dx + (E.nr.w - vaj.y) * dy);
+    const R rho_i = fabs(E.pr.z), rho_j = fabs(prj.z);
+    if (K.lj_on && take && tj > 0 && E.pr.z < R(0) && dist < K.lj_r0) {
         const R rn = K.lj_r0 * rinv;
         const R rn2 = rn * rn, rn4 = rn2 * rn2;
         const R f = K.lj_D * (rn4 * rn4 * rn4 - rn4) * (rinv * rinv);
         S.ljx -= dx * f;
         S.ljy -= dy * f;
     }
-    const R inv_rs = fast_rcp<R>(pri.z + prj.z);
-    R coef = pri.w + prj.w;
-    const R dvx = pri.x - prj.x, dvy = pri.y - prj.y;
+    const R inv_rs = fast_rcp<R>(rho_i + rho_j);
+    R coef = E.pr.w + prj.w;
+    const R dvx = E.pr.x - prj.x, dvy = E.pr.y - prj.y;
     if (K.av_on) {
         const R vr = dvx * dx + dvy * dy;
         const R mu = K.h * vr * fast_rcp<R>(d2 + K.eta2);
@@ -231,18 +380,21 @@ __device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>
     S.sx += fc * dx;
     S.sy += fc * dy;
     if (K.xsph_on) {
-        const R cw = take & (ti > 0) ? W * inv_rs : R(0);
+        const R cw = take & (E.pr.z > R(0)) ? W * inv_rs : R(0);
         S.xx += dvx * cw;
         S.xy += dvy * cw;
     }
 }
 
 // Summation density from the lists (Density.h:25-61): rho = m W(0) +
-// sum m W_ij, p = EOS(rho), into the predicted slots (the forces use this
-// rho) and the pressure array.  Cubic spline (list steps run the cubic
-// configuration).
+// sum m W_ij, p = EOS(rho), into the pred records (the forces use this rho)
+// and the pressure array; renormalised on the steps that ask for it
+// (ModelParams.tex:124-133, rho_i the density before the step).  Cubic
+// spline.  Aliveness and the old densities come from va (pred is being
+// rewritten by this pass).
 template <class R>
 __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
+                                                           const real4<R>* __restrict__ nearr,
                                                            real4<R>* __restrict__ pred, R* __restrict__ pres,
                                                            const uint32_t* __restrict__ nbl,
                                                            const uint32_t* __restrict__ nbc) {
@@ -256,34 +408,53 @@ __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* _
     const real4<R> vaj = X.va[s];
     const R w0 = kernel_w<R>(K, P.skf_density, R(0));
     R rho = m * w0;
-    const bool renorm = ctrl->cur_renorm != 0;  // ModelParams.tex:124-133, rho_i before the step
+    const bool renorm = ctrl->cur_renorm != 0;
     R wsum = renorm ? m / vaj.z * w0 : R(0);
     bool overflow = false;
-    if ((int)vaj.w != 0) {
-        for_each_listed<R>(P, P.max_nbr - 1, s, X.pos[s], X.pos, X.va, pred, nbl, nbc[s], overflow,
-                           [&](bool take, double dx, double dy, const real4<R>& vai, const real4<R>&) {
-                               const R w = take ? kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0)) : R(0);
-                               rho += m * w;
-                               if (renorm) wsum += m / vai.z * w;
-                           });
+    const uint32_t cnt = (int)vaj.w != 0 ? nbc[s] : 0u;
+    if (cnt) {
+        CutFP32 C;
+        C.lo = (float)(P.cut2 * (1.0 - 1e-5));
+        C.hi = (float)(P.cut2 * (1.0 + 1e-5));
+        C.cellf = (float)P.cell;
+        C.far = (float)(4.0 * P.cell);
+        const real4<R> nj = nearr[s];
+        const double2 pj = X.pos[s];
+        const uint32_t cap = P.max_nbr - 1;
+        uint32_t hits = 0;
+#pragma unroll 2
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t e = nbl[(size_t)k * P.n + s];
+            const uint32_t i = e & kSlotMask;
+            const real4<R> vai = X.va[i];
+            R dx, dy;
+            const bool hit = list_cut(P, C, e >> kSlotBits, nearr[i], nj, i, pj, X.pos, dx, dy) && (int)vai.w != 0;
+            const bool take = hit & (hits < cap);
+            overflow |= hit & !take;
+            hits += take ? 1u : 0u;
+            const R w = take ? kernel_w<R>(K, P.skf_density, pair_dist<R>(dx, dy, 0.0)) : R(0);
+            rho += m * w;
+            if (renorm) wsum += m / vai.z * w;
+        }
     }
     if (renorm) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     real4<R> pr = pred[s];
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
-    pr.z = rho;
+    pr.z = typed_rho<R>(rho, (int)vaj.w);
     pr.w = p / (rho * rho);
     pred[s] = pr;
     pres[s] = p;
 }
 
 // Pair pass from the lists: continuity density rate, pressure, artificial
-// viscosity, Lennard-Jones walls and XSPH (the cubic fast path of
-// pair_fast_kernel, the same per-pair arithmetic), the dt maxima, and the
-// fields for the integrator.  With continuity it also writes the pressure
-// of the current density (the reference's p array).
+// viscosity, Lennard-Jones walls and XSPH (the cubic configuration of
+// pair_fast_kernel), the dt maxima, and the fields for the integrator.  With
+// continuity it also writes the pressure of the current density (the
+// reference's p array).
 template <class R>
 __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
+                                                           const real4<R>* __restrict__ nearr,
                                                            const real4<R>* __restrict__ pred, R* __restrict__ pres,
                                                            const uint32_t* __restrict__ nbl,
                                                            const uint32_t* __restrict__ nbc,
@@ -304,19 +475,15 @@ __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* _
     const FastPair<R> K(P);
     PairSums<R> S;
     bool overflow = false;
-    double2 pj = make_double2(0, 0);
     real4<R> vaj{}, prj{};
     if (live) {
-        pj = X.pos[s];
         vaj = X.va[s];
         prj = pred[s];
     }
     const int tj = (int)vaj.w;
     if (live && tj != 0) {
-        for_each_listed<R>(P, P.max_nbr - 1, s, pj, X.pos, X.va, pred, nbl, nbc[s], overflow,
-                           [&](bool take, double dx, double dy, const real4<R>& vai, const real4<R>& pri) {
-                               list_pair_term<R>(K, S, take, (R)dx, (R)dy, vai, pri, vaj, prj, tj);
-                           });
+        for_each_listed<R>(P, P.max_nbr - 1, s, nearr[s], X.pos[s], X.pos, nearr, pred, nbl, nbc[s], overflow,
+                           [&](const ListEntry<R>& E) { list_pair_term<R>(K, S, E, vaj, prj, tj); });
     }
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     const R m = (R)P.mass;
@@ -343,14 +510,17 @@ __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* _
 }
 
 // whole_step of every slot from X into A (in place on list steps), the cell
-// key for the next build, the step's largest displacement, and the
-// prediction of the next step (PredictHalfStep.cpp:16-22 with this step's
-// dt, which is the next step's: (v^, rho^, p / rho^2)).
+// key for the next build, the step's largest displacement, and the records
+// of the next step's pair pass: near (the new position relative to the home
+// cell, the new velocity) and pred (PredictHalfStep.cpp:16-22 with this
+// step's dt, which is the next step's).
 template <class R>
 __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
                                                                 const real4<R>* __restrict__ der,
                                                                 const real2<R>* __restrict__ xs, real4<R>* pred,
-                                                                StateA<R> A, uint32_t* __restrict__ key_next,
+                                                                real4<R>* __restrict__ nearr,
+                                                                const uint32_t* __restrict__ home, StateA<R> A,
+                                                                uint32_t* __restrict__ key_next,
                                                                 const unsigned long long* __restrict__ maxslots,
                                                                 unsigned long long* __restrict__ dslots) {
     Ctrl& c = *ctrl;
@@ -379,7 +549,11 @@ __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ct
         const real4<R> d = der[s];
         const real2<R> x = xs[s];
         const double2 p0 = X.pos[s];
-        const real4<R> pr = P.density_treatment != 0 ? real4<R>{} : pred[s];
+        real4<R> pr{};
+        if (P.density_treatment == 0) {  // the summed density (sign-encoded)
+            pr = pred[s];
+            pr.z = fabs(pr.z);
+        }
         const uint32_t oid = X.oid[s];
         const Stepped2<R> o = whole_step_one<R>(P, c, p0, X.va[s], pr, oid, d.x, d.y, d.z, x.x, x.y, dt, outside);
         A.pos[s] = o.pos;
@@ -395,16 +569,9 @@ __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ct
         key_next[s] = it == 0 ? P.ncells : cell_key(P, o.pos.x, o.pos.y, bad);
         const double mx = o.pos.x - p0.x, my = o.pos.y - p0.y;
         moved = sqrt(mx * mx + my * my);
-        // the next step's prediction, as reorder_kernel computes it
-        const R dtr = (R)dt;
-        const bool cont = P.density_treatment != 0;
-        real4<R> np;
-        np.x = o.va.x + d.x * dtr * R(0.5);
-        np.y = o.va.y + d.y * dtr * R(0.5);
-        np.z = cont ? o.va.z + d.z * dtr * R(0.5) : o.va.z;
-        const R p = eos_p<R>(o.va.z, (R)P.rho0, (R)P.eos_B);
-        np.w = p / (np.z * np.z);
-        pred[s] = np;
+        R p;
+        pred[s] = predict_record<R>(P, o.va, d.x, d.y, d.z, (R)dt, p);
+        nearr[s] = near_record<R>(P, o.pos, o.va.x, o.va.y, home ? home[s] : 0u);
     }
     wave_count_add(&c.outside, outside);
     if (bad) atomicOr(&c.grid_overflow, 1u);

@@ -202,6 +202,8 @@ struct rrsph_ctx {
     bool list_mode = false;
     bool list_suspended = false;        // the adjoint's tapes and replays run without lists
     uint32_t *nbl = nullptr, *nbc = nullptr;
+    void* nearr = nullptr;              // near records of the list pass (real4 per slot)
+    uint32_t* home = nullptr;           // each slot's cell at the last build
     unsigned long long* dslots = nullptr;  // per-step largest displacement (block maxima)
     hipGraph_t graph_list = nullptr, graph_build = nullptr;
     hipGraphExec_t gexec_list = nullptr, gexec_build = nullptr;
@@ -424,34 +426,37 @@ template <class R> void enqueue_list_step_t(rrsph_ctx* c, Profiler* prof, bool b
         list_prologue<<<1, kBlock, 0, c->stream>>>(P, c->ctrl, c->dslots, c->maxslots, build ? 1u : 0u);
     });
     ListState<R> X{c->posA, (const real4<R>*)c->vaA, c->oidA};
+    real4<R>* nearr = (real4<R>*)c->nearr;
     if (build) {
         enqueue_sort(c, c->ctrl, L);
         StateA<R> A = stateA<R>(c);
         StateB<R> B = stateB<R>(c, c->pred, c->pres);
         L(K_REORDER, [&] {
-            reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
-                                                            c->maxslots, nullptr);
+            reorder_list_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B,
+                                                                 nearr, c->home, c->LB, c->maxslots);
         });
         L(K_LIST_BUILD, [&] {
-            list_build_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, c->posB, (const real4<R>*)c->vaB, c->LB,
-                                                                   c->nbl, c->nbc);
+            list_build_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, c->posB, (const real4<R>*)c->vaB,
+                                                                   c->sorted_keys, c->LB, c->nbl, c->nbc);
         });
         X = ListState<R>{c->posB, (const real4<R>*)c->vaB, c->oidB};
     }
     real4<R>* pred = (real4<R>*)c->pred;
     if (P.density_treatment == 0) {
         L(K_DENSITY_SUM, [&] {
-            density_sum_list<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, pred, (R*)c->pres, c->nbl, c->nbc);
+            density_sum_list<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, nearr, pred, (R*)c->pres, c->nbl,
+                                                                  c->nbc);
         });
     }
     L(K_PAIR, [&] {
-        pair_list_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, pred, (R*)c->pres, c->nbl, c->nbc,
-                                                              (real4<R>*)c->der, (real2<R>*)c->xs, c->maxslots);
+        pair_list_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, nearr, pred, (R*)c->pres, c->nbl,
+                                                              c->nbc, (real4<R>*)c->der, (real2<R>*)c->xs,
+                                                              c->maxslots);
     });
     L(K_INTEGRATE, [&] {
         integrate_list_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, X, (const real4<R>*)c->der,
-                                                               (const real2<R>*)c->xs, pred, stateA<R>(c),
-                                                               c->key_next, c->maxslots, c->dslots);
+                                                               (const real2<R>*)c->xs, pred, nearr, c->home,
+                                                               stateA<R>(c), c->key_next, c->maxslots, c->dslots);
     });
 }
 
@@ -799,7 +804,8 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     // neighbour-list steps: the 2D cubic configuration; skin as a fraction of the cut-off
     double skin_frac = 0.125;
     if (const char* e = std::getenv("RRSPH_HIP_LIST_SKIN")) skin_frac = std::strtod(e, nullptr);
-    ctx->list_mode = !d3 && ctx->fast_pair && skin_frac > 0.0 && skin_frac < 0.9;
+    // (an entry holds a slot in kSlotBits bits)
+    ctx->list_mode = !d3 && ctx->fast_pair && skin_frac > 0.0 && skin_frac < 0.9 && P.n <= kSlotMask;
     if (const char* e = std::getenv("RRSPH_HIP_LIST")) ctx->list_mode = ctx->list_mode && std::strcmp(e, "0") != 0;
     P.list_reach = P.cell * (1.0 + skin_frac);
     P.list_r2 = P.list_reach * P.list_reach * (1.0 + 1e-12);
@@ -832,7 +838,7 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->dslots};
+                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->nearr, c->home, c->dslots};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -950,6 +956,8 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     if (ctx->list_mode) {
         ctx->nbl = dalloc<uint32_t>((size_t)ctx->P.list_cap * n, e);
         ctx->nbc = dalloc<uint32_t>(n, e);
+        ctx->nearr = dalloc<double4>(n, e);
+        ctx->home = dalloc<uint32_t>(n, e);
         ctx->dslots = dalloc<unsigned long long>(2 * kMaxSlots, e);
     }
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
