@@ -16,7 +16,8 @@
 // kStatusRebuild); the host re-runs it as a build step.  The host schedules
 // builds ahead from the measured displacement rate, so such halts are rare.
 //
-// What the pair pass gathers per entry, two records in slot order:
+// What the pair pass gathers per entry, one record of two halves per slot
+// (32 bytes in FP32, 64 in FP64, so one cache line):
 //   near  FP32: (xi_x, xi_y, vx, vy), xi the position relative to the
 //         origin of the particle's home cell (its cell at the last build),
 //         rounded from the FP64 position; FP64: (x, y, vx, vy), the FP64
@@ -55,6 +56,12 @@ template <class R> struct ListState {
     const double2* pos;
     const real4<R>* va;
     const uint32_t* oid;
+};
+
+// The record of a slot: near and pred halves.
+template <class R> struct alignas(2 * sizeof(real4<R>)) ListRec {
+    real4<R> nr;
+    real4<R> pr;
 };
 
 // The sign-encoded predicted density of the pred record.
@@ -100,7 +107,7 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) reorder_list_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
                                                               const uint32_t* __restrict__ skey,
                                                               const uint32_t* __restrict__ perm, StateA<R> A,
-                                                              StateB<R> B, real4<R>* __restrict__ nearr,
+                                                              StateB<R> B, ListRec<R>* __restrict__ rec,
                                                               uint32_t* __restrict__ home, uint32_t* __restrict__ LB,
                                                               unsigned long long* __restrict__ maxslots) {
     if (ctrl->halted) return;
@@ -123,9 +130,11 @@ __global__ void __launch_bounds__(kBlock) reorder_list_kernel(StepParams P, cons
         B.pos[s] = pos;
         B.va[s] = va;
         B.oid[s] = oid;
-        B.pred[s] = pr;
         B.pres[s] = p;
-        nearr[s] = near_record<R>(P, pos, va.x, va.y, k);
+        ListRec<R> r;
+        r.nr = near_record<R>(P, pos, va.x, va.y, k);
+        r.pr = pr;
+        rec[s] = r;
         home[s] = k;
     }
     fill_cell_starts(P, s, live, k, skey, LB);
@@ -145,17 +154,20 @@ __global__ void __launch_bounds__(kBlock) list_prologue(StepParams P, Ctrl* __re
     for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off));
     if (__lane_id() == 0) s_m[threadIdx.x / kWave] = d;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        double dmax = s_m[0];
-        for (int w = 1; w < kBlock / kWave; ++w) dmax = fmax(dmax, s_m[w]);
-        Ctrl& c = *ctrl;
-        if (!c.halted && c.pending) {  // the step being committed moved no particle further than dmax
-            c.disp_acc += dmax;
-            c.disp_last = dmax;
-        }
-        const uint32_t valid = c.list_valid;
-        step_prologue_body(P, ctrl);
-        if (!c.halted) {
+    uint32_t valid = 0;
+    block_prologue(
+        P, ctrl,
+        [&](Ctrl& c) {
+            double dmax = s_m[0];
+            for (int w = 1; w < kBlock / kWave; ++w) dmax = fmax(dmax, s_m[w]);
+            if (!c.halted && c.pending) {  // the step being committed moved no particle further than dmax
+                c.disp_acc += dmax;
+                c.disp_last = dmax;
+            }
+            valid = c.list_valid;
+        },
+        [&](Ctrl& c) {
+            if (c.halted) return;
             c.list_valid = build ? 1u : valid;
             if (build) {
                 c.disp_acc = 0.0;
@@ -165,9 +177,7 @@ __global__ void __launch_bounds__(kBlock) list_prologue(StepParams P, Ctrl* __re
                 c.status = kStatusRebuild;
                 if (c.wall_disappear) c.nwm_done = 0;  // the re-run prologue fires it again
             }
-        }
-    }
-    __syncthreads();
+        });
     maxslots[threadIdx.x] = 0;
     maxslots[kMaxSlots + threadIdx.x] = 0;
     dslots[threadIdx.x] = 0;
@@ -203,6 +213,11 @@ __global__ void __launch_bounds__(kBlock) list_build_kernel(StepParams P, Ctrl* 
         const double2 pj = pos[s];
         const uint32_t kj = skey[s];
         const int hx = (int)(kj / (uint32_t)P.ncy), hy = (int)kj - hx * P.ncy;
+        // the list pass takes the FP32 positions relative to the home cell
+        // for within 2.1 cells of its origin: a particle outside the cell
+        // grid (keyed to the nearest edge cell) runs the step without lists
+        const double ox = pj.x - (P.xmin + hx * P.cell), oy = pj.y - (P.ymin + hy * P.cell);
+        bad = !(ox > -P.cell && ox < 2.0 * P.cell && oy > -P.cell && oy < 2.0 * P.cell);
         const double reach = P.list_reach * (1.0 + 1e-9);
         const int x0 = clamped_cell(pj.x - reach, P.xmin, P.cell, P.ncx);
         const int x1 = clamped_cell(pj.x + reach, P.xmin, P.cell, P.ncx);
@@ -248,88 +263,117 @@ template <class R> struct ListEntry {
 };
 
 // r_i - r_j of an entry and whether it is within the cut-off.  FP32: from
-// the near records and the home-cell offset, the reference's FP64 test on
-// the FP64 positions only inside the band around the cut-off or when a
-// position lies more than four cells from its home cell's origin.  FP64:
-// the reference's test on the FP64 positions of the near records.
+// the near records and the offset of the two home cells (a table in LDS
+// indexed by the entry's code), the reference's FP64 test on the FP64
+// positions only inside the band around the cut-off.  The build guarantees
+// that every listed position lies within a cell of its home cell, and the
+// lists go stale before any particle moves half the skin, so xi stays below
+// 2.1 cells and its rounding far inside the band.  FP64: the reference's test
+// on the FP64 positions of the near records.
 struct CutFP32 {
-    float lo, hi, cellf, far;
+    float lo, hi;
 };
-__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32& C, uint32_t code, const float4& ni,
-                                         const float4& nj, uint32_t i, double2 pj, const double2* __restrict__ pos,
-                                         float& dx, float& dy) {
-    dx = (ni.x - nj.x) + (float)((int)(code & 7u) - 3) * C.cellf;
-    dy = (ni.y - nj.y) + (float)((int)(code >> 3) - 3) * C.cellf;
+__device__ __forceinline__ CutFP32 cut_band(const StepParams& P) {
+    CutFP32 C;
+    C.lo = (float)(P.cut2 * (1.0 - 1e-5));
+    C.hi = (float)(P.cut2 * (1.0 + 1e-5));
+    return C;
+}
+__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32& C, const float2* s_off, uint32_t e,
+                                         const float4& ni, const float4& nj, double2 pj,
+                                         const double2* __restrict__ pos, float& dx, float& dy) {
+    const float2 o = s_off[e >> kSlotBits];
+    dx = (ni.x - nj.x) + o.x;
+    dy = (ni.y - nj.y) + o.y;
     const float d2 = dx * dx + dy * dy;
-    const bool far = fmaxf(fabsf(ni.x), fabsf(ni.y)) > C.far || fmaxf(fabsf(nj.x), fabsf(nj.y)) > C.far;
-    if (!far && !(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
-    const double2 q = pos[i];  // rare: the band around the cut-off
+    if (!(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
+    const double2 q = pos[e & kSlotMask];  // rare: the band around the cut-off
     const double ddx = __dsub_rn(q.x, pj.x), ddy = __dsub_rn(q.y, pj.y);
     dx = (float)ddx;
     dy = (float)ddy;
     return dist2_exact(ddx, ddy) < P.cut2;
 }
-__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32&, uint32_t, const double4& ni,
-                                         const double4& nj, uint32_t, double2, const double2* __restrict__,
+__device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32&, const float2*, uint32_t,
+                                         const double4& ni, const double4& nj, double2, const double2* __restrict__,
                                          double& dx, double& dy) {
     dx = __dsub_rn(ni.x, nj.x);
     dy = __dsub_rn(ni.y, nj.y);
     return dist2_exact(dx, dy) < P.cut2;
 }
 
+// The table of home-cell offsets of the entry codes, code = (dcx + 3) |
+// (dcy + 3) << 3 -> (dcx cell, dcy cell) rounded to FP32, filled by the
+// first 64 threads of the block (a barrier must follow).
+__device__ __forceinline__ void fill_cell_offsets(const StepParams& P, float2* s_off) {
+    if (threadIdx.x < 64) {
+        const int c = (int)threadIdx.x;
+        s_off[c] = make_float2((float)(((c & 7) - 3) * P.cell), (float)(((c >> 3) - 3) * P.cell));
+    }
+}
+
+// Entry k of slot s's list: the lists hold list_cap + kListPad rows, zero
+// where no build wrote (so the loads of the entries after a list's last one
+// stay in bounds and name valid slots), and the byte offsets fit 32 bits.
+constexpr uint32_t kListPad = 8;
+__device__ __forceinline__ uint32_t list_entry(const uint32_t* __restrict__ nbl, uint32_t byte_off) {
+    return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const char*>(nbl) + byte_off);
+}
+
 // Visits the entries of slot s's list: f(ListEntry) for every entry (f must
 // add nothing when take is false: every entry runs the same instructions,
-// most entries being neighbours).  Entries beyond the first `cap`
-// neighbours set `overflow` (GridFind.cpp:158-188).  Two register sets in
-// turn: while one entry is processed, the next one's records and the index
-// after it are in flight; each index is loaded before the gathers issued
-// with it.
-template <class R, class F>
-__device__ __forceinline__ void for_each_listed(const StepParams& P, uint32_t cap, uint32_t s,
+// most entries being neighbours).  kCap: count the neighbours, and entries
+// beyond the first `cap` set `overflow` (GridFind.cpp:158-188); a wave
+// whose lists are all no longer than cap cannot overflow and skips the
+// count.  Entries go in groups of G: the group's records are gathered
+// together and the next group's entries loaded behind them, so that no
+// gathered value lives across iterations (the loop waits only on its own
+// loads) and 2 G gathers are in flight at the first entry's use.
+template <class R, bool kCap, class F>
+__device__ __forceinline__ void for_each_listed(const StepParams& P, const float2* s_off, uint32_t cap, uint32_t s,
                                                 const real4<R>& nj, double2 pj, const double2* __restrict__ pos,
-                                                const real4<R>* __restrict__ nearr, const real4<R>* __restrict__ pred,
-                                                const uint32_t* __restrict__ nbl, uint32_t cnt, bool& overflow,
+                                                const ListRec<R>* __restrict__ rec, const uint32_t* __restrict__ nbl,
+                                                uint32_t cnt, bool& overflow,
                                                 F&& f) {
+    constexpr uint32_t G = sizeof(R) == 4 ? 4 : 2;
+    static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
     if (cnt == 0) return;
-    const size_t n = P.n;
-    const uint32_t last = cnt - 1;
-    CutFP32 C;
-    C.lo = (float)(P.cut2 * (1.0 - 1e-5));
-    C.hi = (float)(P.cut2 * (1.0 + 1e-5));
-    C.cellf = (float)P.cell;
-    C.far = (float)(4.0 * P.cell);
+    const uint32_t row = P.n * 4u;
+    uint32_t off = s * 4u;
+    const CutFP32 C = cut_band(P);
     uint32_t hits = 0;
     bool over = false;
-    auto visit = [&](uint32_t e, const real4<R>& ni, const real4<R>& pi) {
-        ListEntry<R> E;
-        const uint32_t i = e & kSlotMask;
-        const bool in = list_cut(P, C, e >> kSlotBits, ni, nj, i, pj, pos, E.dx, E.dy);
-        const bool hit = in & (pi.z != R(0));
-        E.take = hit & (hits < cap);
-        over |= hit & !E.take;
-        hits += E.take ? 1u : 0u;
-        E.nr = ni;
-        E.pr = pi;
-        f(E);
-    };
-    uint32_t eA = nbl[s];
-    uint32_t eB = nbl[min(1u, last) * n + s];
-    real4<R> nA = nearr[eA & kSlotMask], pA = pred[eA & kSlotMask], nB, pB;
-    uint32_t k = 0;
+    uint32_t e[G];
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, off + u * row);
+    off += G * row;
 #pragma unroll 1
-    for (; k + 1 < cnt; k += 2) {
-        const uint32_t eA2 = nbl[min(k + 2, last) * n + s];
-        nB = nearr[eB & kSlotMask];
-        pB = pred[eB & kSlotMask];
-        visit(eA, nA, pA);
-        eA = eA2;
-        const uint32_t eB2 = nbl[min(k + 3, last) * n + s];
-        nA = nearr[eA & kSlotMask];
-        pA = pred[eA & kSlotMask];
-        visit(eB, nB, pB);
-        eB = eB2;
+    for (uint32_t k = 0; k < cnt; k += G) {
+        ListRec<R> r[G];
+        uint32_t en[G];
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) r[u] = rec[e[u] & kSlotMask];
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, off + u * row);
+        off += G * row;
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) {
+            ListEntry<R> E;
+            const bool in = list_cut(P, C, s_off, e[u], r[u].nr, nj, pj, pos, E.dx, E.dy);
+            const bool hit = in & (r[u].pr.z != R(0)) & (k + u < cnt);
+            if (kCap) {
+                E.take = hit & (hits < cap);
+                over |= hit & !E.take;
+                hits += E.take ? 1u : 0u;
+            } else {
+                E.take = hit;
+            }
+            E.nr = r[u].nr;
+            E.pr = r[u].pr;
+            f(E);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) e[u] = en[u];
     }
-    if (k < cnt) visit(eA, nA, pA);
     overflow |= over;
 }
 
@@ -380,7 +424,7 @@ __device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>
     S.sx += fc * dx;
     S.sy += fc * dy;
     if (K.xsph_on) {
-        const R cw = take & (E.pr.z > R(0)) ? W * inv_rs : R(0);
+        const R cw = (take & (E.pr.z > R(0))) ? W * inv_rs : R(0);
         S.xx += dvx * cw;
         S.xy += dvy * cw;
     }
@@ -394,13 +438,15 @@ __device__ __forceinline__ void list_pair_term(const FastPair<R>& K, PairSums<R>
 // rewritten by this pass).
 template <class R>
 __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
-                                                           const real4<R>* __restrict__ nearr,
-                                                           real4<R>* __restrict__ pred, R* __restrict__ pres,
+                                                           ListRec<R>* __restrict__ rec, R* __restrict__ pres,
                                                            const uint32_t* __restrict__ nbl,
                                                            const uint32_t* __restrict__ nbc) {
     if (ctrl->halted) return;
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
+    __shared__ float2 s_off[64];
+    fill_cell_offsets(P, s_off);
+    __syncthreads();
     const uint32_t s = lb * kBlock + threadIdx.x;
     if (s >= P.n) return;
     const KernelConsts<R> K = kernel_consts<R>(P);
@@ -413,22 +459,20 @@ __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* _
     bool overflow = false;
     const uint32_t cnt = (int)vaj.w != 0 ? nbc[s] : 0u;
     if (cnt) {
-        CutFP32 C;
-        C.lo = (float)(P.cut2 * (1.0 - 1e-5));
-        C.hi = (float)(P.cut2 * (1.0 + 1e-5));
-        C.cellf = (float)P.cell;
-        C.far = (float)(4.0 * P.cell);
-        const real4<R> nj = nearr[s];
+        const CutFP32 C = cut_band(P);
+        const real4<R> nj = rec[s].nr;
         const double2 pj = X.pos[s];
         const uint32_t cap = P.max_nbr - 1;
+        const uint32_t row = P.n * 4u;
+        uint32_t off = s * 4u;
         uint32_t hits = 0;
 #pragma unroll 2
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t e = nbl[(size_t)k * P.n + s];
+        for (uint32_t k = 0; k < cnt; ++k, off += row) {
+            const uint32_t e = list_entry(nbl, off);
             const uint32_t i = e & kSlotMask;
             const real4<R> vai = X.va[i];
             R dx, dy;
-            const bool hit = list_cut(P, C, e >> kSlotBits, nearr[i], nj, i, pj, X.pos, dx, dy) && (int)vai.w != 0;
+            const bool hit = list_cut(P, C, s_off, e, rec[i].nr, nj, pj, X.pos, dx, dy) && (int)vai.w != 0;
             const bool take = hit & (hits < cap);
             overflow |= hit & !take;
             hits += take ? 1u : 0u;
@@ -439,11 +483,11 @@ __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* _
     }
     if (renorm) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
-    real4<R> pr = pred[s];
+    real4<R> pr = rec[s].pr;
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
     pr.z = typed_rho<R>(rho, (int)vaj.w);
     pr.w = p / (rho * rho);
-    pred[s] = pr;
+    rec[s].pr = pr;
     pres[s] = p;
 }
 
@@ -454,8 +498,7 @@ __global__ void __launch_bounds__(kBlock) density_sum_list(StepParams P, Ctrl* _
 // reference's p array).
 template <class R>
 __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
-                                                           const real4<R>* __restrict__ nearr,
-                                                           const real4<R>* __restrict__ pred, R* __restrict__ pres,
+                                                           const ListRec<R>* __restrict__ rec, R* __restrict__ pres,
                                                            const uint32_t* __restrict__ nbl,
                                                            const uint32_t* __restrict__ nbc,
                                                            real4<R>* __restrict__ der, real2<R>* __restrict__ xs,
@@ -470,6 +513,9 @@ __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* _
     }
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
+    __shared__ float2 s_off[64];
+    fill_cell_offsets(P, s_off);
+    __syncthreads();
     const uint32_t s = lb * kBlock + threadIdx.x;
     const bool live = s < P.n;
     const FastPair<R> K(P);
@@ -478,12 +524,19 @@ __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* _
     real4<R> vaj{}, prj{};
     if (live) {
         vaj = X.va[s];
-        prj = pred[s];
+        prj = rec[s].pr;
     }
     const int tj = (int)vaj.w;
-    if (live && tj != 0) {
-        for_each_listed<R>(P, P.max_nbr - 1, s, nearr[s], X.pos[s], X.pos, nearr, pred, nbl, nbc[s], overflow,
-                           [&](const ListEntry<R>& E) { list_pair_term<R>(K, S, E, vaj, prj, tj); });
+    const uint32_t cnt = live && tj != 0 ? nbc[s] : 0u;
+    const uint32_t cap = P.max_nbr - 1;
+    if (cnt) {
+        auto term = [&](const ListEntry<R>& E) { list_pair_term<R>(K, S, E, vaj, prj, tj); };
+        const real4<R> nj = rec[s].nr;
+        const double2 pj = X.pos[s];
+        if (__ballot(cnt > cap))  // some list of the wave may hold more than cap neighbours
+            for_each_listed<R, true>(P, s_off, cap, s, nj, pj, X.pos, rec, nbl, cnt, overflow, term);
+        else
+            for_each_listed<R, false>(P, s_off, cap, s, nj, pj, X.pos, rec, nbl, cnt, overflow, term);
     }
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     const R m = (R)P.mass;
@@ -517,8 +570,8 @@ __global__ void __launch_bounds__(kBlock) pair_list_kernel(StepParams P, Ctrl* _
 template <class R>
 __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ctrl* __restrict__ ctrl, ListState<R> X,
                                                                 const real4<R>* __restrict__ der,
-                                                                const real2<R>* __restrict__ xs, real4<R>* pred,
-                                                                real4<R>* __restrict__ nearr,
+                                                                const real2<R>* __restrict__ xs,
+                                                                ListRec<R>* __restrict__ rec,
                                                                 const uint32_t* __restrict__ home, StateA<R> A,
                                                                 uint32_t* __restrict__ key_next,
                                                                 const unsigned long long* __restrict__ maxslots,
@@ -551,7 +604,7 @@ __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ct
         const double2 p0 = X.pos[s];
         real4<R> pr{};
         if (P.density_treatment == 0) {  // the summed density (sign-encoded)
-            pr = pred[s];
+            pr = rec[s].pr;
             pr.z = fabs(pr.z);
         }
         const uint32_t oid = X.oid[s];
@@ -570,8 +623,10 @@ __global__ void __launch_bounds__(kBlock) integrate_list_kernel(StepParams P, Ct
         const double mx = o.pos.x - p0.x, my = o.pos.y - p0.y;
         moved = sqrt(mx * mx + my * my);
         R p;
-        pred[s] = predict_record<R>(P, o.va, d.x, d.y, d.z, (R)dt, p);
-        nearr[s] = near_record<R>(P, o.pos, o.va.x, o.va.y, home ? home[s] : 0u);
+        ListRec<R> r;
+        r.pr = predict_record<R>(P, o.va, d.x, d.y, d.z, (R)dt, p);
+        r.nr = near_record<R>(P, o.pos, o.va.x, o.va.y, home[s]);
+        rec[s] = r;
     }
     wave_count_add(&c.outside, outside);
     if (bad) atomicOr(&c.grid_overflow, 1u);

@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(kResortBlock) resort_sort_small(StepParams P, 
         }
     };
     load_pass(0);  // in flight during the prologue
-    if (tid == 0) step_prologue_body(P, ctrl);  // published by the barriers below
+    block_prologue(P, ctrl, [](Ctrl&) {}, [](Ctrl&) {});
     uint32_t run = 0;
     for (uint32_t base = 0; base < nw; base += kResortPass) {
         if (base != 0) load_pass(base);

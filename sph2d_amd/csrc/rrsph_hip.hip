@@ -202,7 +202,7 @@ struct rrsph_ctx {
     bool list_mode = false;
     bool list_suspended = false;        // the adjoint's tapes and replays run without lists
     uint32_t *nbl = nullptr, *nbc = nullptr;
-    void* nearr = nullptr;              // near records of the list pass (real4 per slot)
+    void* rec = nullptr;                // records of the list pass (ListRec per slot)
     uint32_t* home = nullptr;           // each slot's cell at the last build
     unsigned long long* dslots = nullptr;  // per-step largest displacement (block maxima)
     hipGraph_t graph_list = nullptr, graph_build = nullptr;
@@ -323,7 +323,7 @@ template <class R> void enqueue_step_t(rrsph_ctx* c, Profiler* prof, bool full_s
     const uint32_t nb = grid_blocks(P.n, kBlock);
     const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
     if (full_sort) {
-        L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+        L(K_PROLOGUE, [&] { step_prologue<<<1, kWave, 0, c->stream>>>(P, c->ctrl); });
         enqueue_sort(c, c->ctrl, L);
     } else {
         enqueue_resort(c, L);
@@ -380,7 +380,7 @@ template <class R> void enqueue_step3_t(rrsph_ctx* c, Profiler* prof, bool full_
     const uint32_t nb = grid_blocks(P.n, kBlock);
     const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
     if (full_sort) {
-        L(K_PROLOGUE, [&] { step_prologue<<<1, 1, 0, c->stream>>>(P, c->ctrl); });
+        L(K_PROLOGUE, [&] { step_prologue<<<1, kWave, 0, c->stream>>>(P, c->ctrl); });
         enqueue_sort(c, c->ctrl, L);
     } else {
         enqueue_resort(c, L);
@@ -426,14 +426,14 @@ template <class R> void enqueue_list_step_t(rrsph_ctx* c, Profiler* prof, bool b
         list_prologue<<<1, kBlock, 0, c->stream>>>(P, c->ctrl, c->dslots, c->maxslots, build ? 1u : 0u);
     });
     ListState<R> X{c->posA, (const real4<R>*)c->vaA, c->oidA};
-    real4<R>* nearr = (real4<R>*)c->nearr;
+    ListRec<R>* rec = (ListRec<R>*)c->rec;
     if (build) {
         enqueue_sort(c, c->ctrl, L);
         StateA<R> A = stateA<R>(c);
         StateB<R> B = stateB<R>(c, c->pred, c->pres);
         L(K_REORDER, [&] {
             reorder_list_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B,
-                                                                 nearr, c->home, c->LB, c->maxslots);
+                                                                 rec, c->home, c->LB, c->maxslots);
         });
         L(K_LIST_BUILD, [&] {
             list_build_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, c->posB, (const real4<R>*)c->vaB,
@@ -441,22 +441,19 @@ template <class R> void enqueue_list_step_t(rrsph_ctx* c, Profiler* prof, bool b
         });
         X = ListState<R>{c->posB, (const real4<R>*)c->vaB, c->oidB};
     }
-    real4<R>* pred = (real4<R>*)c->pred;
     if (P.density_treatment == 0) {
         L(K_DENSITY_SUM, [&] {
-            density_sum_list<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, nearr, pred, (R*)c->pres, c->nbl,
-                                                                  c->nbc);
+            density_sum_list<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, rec, (R*)c->pres, c->nbl, c->nbc);
         });
     }
     L(K_PAIR, [&] {
-        pair_list_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, nearr, pred, (R*)c->pres, c->nbl,
-                                                              c->nbc, (real4<R>*)c->der, (real2<R>*)c->xs,
-                                                              c->maxslots);
+        pair_list_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, rec, (R*)c->pres, c->nbl, c->nbc,
+                                                              (real4<R>*)c->der, (real2<R>*)c->xs, c->maxslots);
     });
     L(K_INTEGRATE, [&] {
         integrate_list_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, X, (const real4<R>*)c->der,
-                                                               (const real2<R>*)c->xs, pred, nearr, c->home,
-                                                               stateA<R>(c), c->key_next, c->maxslots, c->dslots);
+                                                               (const real2<R>*)c->xs, rec, c->home, stateA<R>(c),
+                                                               c->key_next, c->maxslots, c->dslots);
     });
 }
 
@@ -811,6 +808,8 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     P.list_r2 = P.list_reach * P.list_reach * (1.0 + 1e-12);
     P.skin_half = 0.5 * P.cell * skin_frac * (1.0 - 1e-9);
     P.list_cap = (uint32_t)std::ceil(p.max_neighbours * (1.0 + skin_frac) * (1.0 + skin_frac)) + 8;
+    // the list pass addresses the lists (list_cap + kListPad rows) with 32-bit byte offsets
+    if ((uint64_t)(P.list_cap + kListPad) * P.n * 4u > 0xffffffffull) ctx->list_mode = false;
     if (d3 && !ctx->fast_pair)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: 3D supports cubic kernels for every term, approximation 2, no artificial pressure, "
@@ -838,7 +837,7 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->nearr, c->home, c->dslots};
+                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->rec, c->home, c->dslots};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -954,14 +953,15 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
     ctx->wpre = dalloc<uint32_t>(n / kWave + 1, e);
     ctx->P.chg_mask = ctx->chg_mask;
     if (ctx->list_mode) {
-        ctx->nbl = dalloc<uint32_t>((size_t)ctx->P.list_cap * n, e);
+        ctx->nbl = dalloc<uint32_t>((size_t)(ctx->P.list_cap + kListPad) * n, e);
         ctx->nbc = dalloc<uint32_t>(n, e);
-        ctx->nearr = dalloc<double4>(n, e);
+        ctx->rec = dalloc<ListRec<double>>(n, e);
         ctx->home = dalloc<uint32_t>(n, e);
         ctx->dslots = dalloc<unsigned long long>(2 * kMaxSlots, e);
     }
     if (e != hipSuccess) return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     if (ctx->dslots) HIP_TRY(hipMemset(ctx->dslots, 0, 2 * kMaxSlots * sizeof(unsigned long long)));
+    if (ctx->nbl) HIP_TRY(hipMemset(ctx->nbl, 0, (size_t)(ctx->P.list_cap + kListPad) * n * sizeof(uint32_t)));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_ctrl, sizeof(Ctrl), hipHostMallocDefault));
     HIP_TRY(hipHostMalloc((void**)&ctx->host_eta, kMaxGauges * sizeof(double), hipHostMallocDefault));
     HIP_TRY(scratch_address(ctx->device, &ctx->scratch));

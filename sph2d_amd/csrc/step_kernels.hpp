@@ -150,9 +150,11 @@ __device__ __forceinline__ void slots_max2(const unsigned long long* __restrict_
 }
 
 // One thread: commit the previous step, decide whether to halt, snapshot.
-__device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ctrl) {
-    Ctrl& c = *ctrl;
-    if (c.halted) return;
+// The commit half of the prologue (one thread, on the control block c):
+// the previous step's time advance, the halts, and this step's snapshot.
+// Returns false when the step halts.
+__device__ __forceinline__ bool prologue_commit(const StepParams& P, Ctrl& c) {
+    if (c.halted) return false;
     c.list_valid = 0;  // a sorted step reorders the slots; a list build sets it again
     if (c.pending) {
         if (P.consistency_check && c.nbr_overflow) {
@@ -165,7 +167,7 @@ __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ct
             if (c.wall_disappear) c.nwm_done = 0;
             c.halted = 1;
             c.status = 3;  // RRSPH_ERR_TOO_MANY_NEIGHBOURS
-            return;
+            return false;
         }
         c.time += c.dt_new;
         c.dt = c.dt_new;
@@ -174,13 +176,13 @@ __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ct
         if (P.consistency_check && c.outside > 0 && P.consistency_treatment == 1) {
             c.halted = 1;
             c.status = 5;  // RRSPH_ERR_OUTSIDE_BOUNDARIES
-            return;
+            return false;
         }
     }
     if (c.time > c.t_end) {  // loop condition time <= simulation_time
         c.halted = 1;
         c.status = 0;
-        return;
+        return false;
     }
     c.cur_time = c.time;
     c.cur_dt = c.dt;
@@ -197,17 +199,77 @@ __device__ __forceinline__ void step_prologue_body(const StepParams& P, Ctrl* ct
     }
     const bool nwm_on = P.boundary_motion && P.nwm != 0 && c.time >= P.nwm_time_start;
     c.piston_active = nwm_on && P.nwm != 4 && P.nwm_axis >= 0;
-    c.piston_v = c.piston_active ? piston_velocity(P, c.time) : 0.0;
-    for (int g = 0; g < kMaxSegments; ++g)
-        c.piston_vseg[g] = c.piston_active && g < P.piston_segments ? piston_velocity(P, c.time, g) : 0.0;
     c.wall_disappear = nwm_on && P.nwm == 4 && !c.nwm_done;
     if (c.wall_disappear) c.nwm_done = 1;
     c.amax_bits = 0;
     c.mumax_bits = 0;
     c.outside = 0;
+    return true;
 }
 
-__global__ void step_prologue(StepParams P, Ctrl* ctrl) { step_prologue_body(P, ctrl); }
+// Term k of paddle seg of the Fourier piston's velocity at time t (k from 1).
+__device__ __forceinline__ double fourier_term(const StepParams& P, double t, int seg, int k) {
+    const double kw = k * P.piston_omega;
+    double sn, cs;
+    sincos(kw * t, &sn, &cs);
+    return kw * (P.piston_a[seg * P.piston_nharm + k - 1] * cs - P.piston_b[seg * P.piston_nharm + k - 1] * sn);
+}
+
+// The step prologue, run by every thread of one block on an LDS copy of the
+// control block: the block loads it and writes it back in one round of
+// coalesced accesses (a chain of dependent global accesses from one thread
+// costs about half a microsecond each), and the Fourier piston's terms are
+// evaluated one per thread, then summed by thread 0 in harmonic order.
+// `before` and `after` run on thread 0 around the commit, on the copy.
+constexpr int kPistonTerms = 32;  // paddles x harmonics of the Fourier piston
+template <class B, class A>
+__device__ __forceinline__ void block_prologue(const StepParams& P, Ctrl* ctrl, B&& before, A&& after) {
+    static_assert(sizeof(Ctrl) % 8 == 0, "the control block is copied in 8-byte words");
+    __shared__ unsigned long long s_cw[sizeof(Ctrl) / 8];
+    __shared__ double s_term[kPistonTerms];
+    __shared__ uint32_t s_go;
+    Ctrl& c = *reinterpret_cast<Ctrl*>(s_cw);
+    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(ctrl);
+    for (uint32_t w = threadIdx.x; w < sizeof(Ctrl) / 8; w += blockDim.x) s_cw[w] = src[w];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        before(c);
+        s_go = prologue_commit(P, c) ? 1u : 0u;
+    }
+    __syncthreads();
+    const bool fourier = s_go && c.piston_active && P.nwm == 6;
+    const int nterms = max(P.piston_segments, 1) * P.piston_nharm;
+    if (fourier)
+        for (int t = threadIdx.x; t < nterms; t += blockDim.x)
+            s_term[t] = fourier_term(P, c.time, t / P.piston_nharm, t % P.piston_nharm + 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (s_go) {
+            double vseg[kMaxSegments];
+            for (int g = 0; g < kMaxSegments; ++g) vseg[g] = 0.0;
+            if (fourier) {
+                for (int g = 0; g < max(P.piston_segments, 1); ++g) {
+                    double v = 0;
+                    for (int k = 0; k < P.piston_nharm; ++k) v += s_term[g * P.piston_nharm + k];
+                    vseg[g] = v;
+                }
+            } else if (c.piston_active) {
+                vseg[0] = piston_velocity(P, c.time);
+            }
+            c.piston_v = vseg[0];
+            for (int g = 0; g < kMaxSegments; ++g) c.piston_vseg[g] = g < P.piston_segments ? vseg[g] : 0.0;
+        }
+        after(c);
+    }
+    __syncthreads();
+    unsigned long long* dst = reinterpret_cast<unsigned long long*>(ctrl);
+    for (uint32_t w = threadIdx.x; w < sizeof(Ctrl) / 8; w += blockDim.x) dst[w] = s_cw[w];
+}
+
+// The prologue of a sorted step (one block of kWave threads).
+__global__ void __launch_bounds__(kWave) step_prologue(StepParams P, Ctrl* ctrl) {
+    block_prologue(P, ctrl, [](Ctrl&) {}, [](Ctrl&) {});
+}
 
 // Cell-start table over keys 0 .. ncells (+1 sentinel entry):
 // LB[c] = first sorted index whose key >= c.  Lane s owns the cells
