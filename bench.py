@@ -536,7 +536,7 @@ def main():
         "dominant_by_time": dom,
         "per_step_ms": {k: round(v, 5) for k, v in per_step.items()},
     }
-    traffic_file = os.path.join(ROOT, "profiles", "pair_kernel_c1_1m_fp32_traffic.json")
+    traffic_file = os.path.join(ROOT, "profiles", "pair_pass_c1_1m_fp32_traffic.json")
     if os.path.exists(traffic_file):
         with open(traffic_file) as f:
             prof = json.load(f)
