@@ -150,13 +150,14 @@ __global__ void __launch_bounds__(kBlock) list_prologue(StepParams P, Ctrl* __re
                                                         unsigned long long* __restrict__ dslots,
                                                         unsigned long long* __restrict__ maxslots, uint32_t build) {
     __shared__ double s_m[kBlock / kWave];
+    const unsigned long long word = ctrl_word(ctrl);
     double d = dbits(dslots[threadIdx.x]);
     for (int off = 32; off > 0; off >>= 1) d = fmax(d, __shfl_xor(d, off));
     if (__lane_id() == 0) s_m[threadIdx.x / kWave] = d;
     __syncthreads();
     uint32_t valid = 0;
     block_prologue(
-        P, ctrl,
+        P, ctrl, word,
         [&](Ctrl& c) {
             double dmax = s_m[0];
             for (int w = 1; w < kBlock / kWave; ++w) dmax = fmax(dmax, s_m[w]);

@@ -126,8 +126,9 @@ __global__ void __launch_bounds__(kResortBlock) resort_sort_small(StepParams P, 
             m[j] = w < nw ? mask[w] : 0ull;
         }
     };
+    const unsigned long long word = ctrl_word(ctrl);
     load_pass(0);  // in flight during the prologue
-    block_prologue(P, ctrl, [](Ctrl&) {}, [](Ctrl&) {});
+    block_prologue(P, ctrl, word, [](Ctrl&) {}, [](Ctrl&) {});
     uint32_t run = 0;
     for (uint32_t base = 0; base < nw; base += kResortPass) {
         if (base != 0) load_pass(base);

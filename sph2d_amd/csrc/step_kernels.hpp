@@ -222,15 +222,24 @@ __device__ __forceinline__ double fourier_term(const StepParams& P, double t, in
 // evaluated one per thread, then summed by thread 0 in harmonic order.
 // `before` and `after` run on thread 0 around the commit, on the copy.
 constexpr int kPistonTerms = 32;  // paddles x harmonics of the Fourier piston
+constexpr uint32_t kCtrlWords = sizeof(Ctrl) / 8;
+static_assert(sizeof(Ctrl) % 8 == 0 && kCtrlWords <= kWave, "the control block is copied in 8-byte words");
+
+// Word threadIdx.x of the control block (the first kCtrlWords threads), to
+// be handed to block_prologue; a kernel loads it first so that the load is
+// in flight with its other loads.
+__device__ __forceinline__ unsigned long long ctrl_word(const Ctrl* ctrl) {
+    return threadIdx.x < kCtrlWords ? reinterpret_cast<const unsigned long long*>(ctrl)[threadIdx.x] : 0ull;
+}
+
 template <class B, class A>
-__device__ __forceinline__ void block_prologue(const StepParams& P, Ctrl* ctrl, B&& before, A&& after) {
-    static_assert(sizeof(Ctrl) % 8 == 0, "the control block is copied in 8-byte words");
-    __shared__ unsigned long long s_cw[sizeof(Ctrl) / 8];
+__device__ __forceinline__ void block_prologue(const StepParams& P, Ctrl* ctrl, unsigned long long word, B&& before,
+                                               A&& after) {
+    __shared__ unsigned long long s_cw[kCtrlWords];
     __shared__ double s_term[kPistonTerms];
     __shared__ uint32_t s_go;
     Ctrl& c = *reinterpret_cast<Ctrl*>(s_cw);
-    const unsigned long long* src = reinterpret_cast<const unsigned long long*>(ctrl);
-    for (uint32_t w = threadIdx.x; w < sizeof(Ctrl) / 8; w += blockDim.x) s_cw[w] = src[w];
+    if (threadIdx.x < kCtrlWords) s_cw[threadIdx.x] = word;
     __syncthreads();
     if (threadIdx.x == 0) {
         before(c);
@@ -262,13 +271,12 @@ __device__ __forceinline__ void block_prologue(const StepParams& P, Ctrl* ctrl, 
         after(c);
     }
     __syncthreads();
-    unsigned long long* dst = reinterpret_cast<unsigned long long*>(ctrl);
-    for (uint32_t w = threadIdx.x; w < sizeof(Ctrl) / 8; w += blockDim.x) dst[w] = s_cw[w];
+    if (threadIdx.x < kCtrlWords) reinterpret_cast<unsigned long long*>(ctrl)[threadIdx.x] = s_cw[threadIdx.x];
 }
 
 // The prologue of a sorted step (one block of kWave threads).
 __global__ void __launch_bounds__(kWave) step_prologue(StepParams P, Ctrl* ctrl) {
-    block_prologue(P, ctrl, [](Ctrl&) {}, [](Ctrl&) {});
+    block_prologue(P, ctrl, ctrl_word(ctrl), [](Ctrl&) {}, [](Ctrl&) {});
 }
 
 // Cell-start table over keys 0 .. ncells (+1 sentinel entry):
