@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+FP32_VECTOR_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 VALU_SIMDS = 1024      # 256 CUs x 4 SIMDs
 CLOCK_HZ = 2.4e9       # MI355X peak engine clock
 # Algorithmic bytes per particle for the pair pass (continuity, XSPH, AV,
@@ -551,6 +552,16 @@ def main():
             valu_ms = valu * 4 / (VALU_SIMDS * CLOCK_HZ) * 1e3
             roofline["valu_issue_bound_ms"] = round(valu_ms, 5)
             roofline["valu_issue_frac"] = round(valu_ms / pair_ms, 4)
+        # FP32 VALU throughput of the pair pass: the profiled floating-point
+        # operations of one launch (masked-off lanes included, an upper
+        # bound) over this run's average launch time, against the vector
+        # peak (157.3 TFLOP/s with packed FP32, which this build does not
+        # emit; 78.6 without)
+        flop = prof.get("fp32_flop_per_launch")
+        if flop:
+            tf = flop / (pair_ms * 1e-3) / 1e12
+            roofline["valu_fp32_tflops"] = round(tf, 2)
+            roofline["valu_fp32_frac_of_peak"] = round(tf / FP32_VECTOR_PEAK_TFLOPS, 4)
 
     roofline["copy_measured_gbs"] = round(copy_bandwidth(), 1)
     roofline["frac_of_measured_copy"] = round(achieved / roofline["copy_measured_gbs"], 4)
