@@ -1,0 +1,515 @@
+// Neighbour-list steps in 3D: the design of list_step.hpp (builds that sort
+// the cells and list every particle within k h + skin, list steps that
+// decide each entry with the reference's cut-off test, a device-side
+// staleness halt) with three-dimensional records.
+//
+// Per slot, in slot order:
+//   rec.nr  FP32: (xi_x, xi_y, xi_z, p / rho^2), xi the position relative to
+//           the origin of the home cell; FP64: (x, y, z, p / rho^2)
+//   rec.pr  (v^x, v^y, v^z, s rho^), s = +1 fluid, -1 boundary, 0 dead
+//   dens    (xi or x, s rho), rho the density before the step: what the
+//           summation density pass gathers (its own record, so that the
+//           pass can write the predicted density and p / rho^2 of its slot
+//           into rec while others read dens)
+// An entry holds the neighbour's slot in its low 25 bits and the offset of
+// the two home cells, (dcx + 2) + 5 (dcy + 2) + 25 (dcz + 2), in its high 7.
+// The FP32 cut-off test takes r_i - r_j = xi_i - xi_j + dc cell, the
+// offsets from a table in LDS, and the FP64 positions and the reference's
+// test (GridFind.cpp:144-176, dist2_exact3) within 1e-5 of the cut-off.
+//
+// Step kernels: list_prologue -> [density_sum_list3] -> pair_list3_kernel
+// -> integrate_list3_kernel (in place).  A build step runs list_prologue ->
+// radix sort -> reorder_list3_kernel -> list_build3_kernel -> the same step
+// kernels on the reordered copy, integrating into A.
+#pragma once
+
+#include "list_step.hpp"
+#include "step3d.hpp"
+
+namespace rrsph {
+
+constexpr uint32_t kSlotBits3 = 25;
+constexpr uint32_t kSlotMask3 = (1u << kSlotBits3) - 1;  // 3D list steps take fewer than 2^25 particles
+constexpr uint32_t kCodes3 = 125;
+
+template <class R> struct ListState3 {
+    const double4* pos;
+    const real4<R>* va;
+    const uint32_t* oid;
+};
+
+// Origin of home cell `home` (a key (cx ncy + cy) ncz + cz).
+__device__ __forceinline__ double3 cell_origin3(const StepParams& P, uint32_t home) {
+    const uint32_t cz = home % (uint32_t)P.ncz;
+    const uint32_t cxy = home / (uint32_t)P.ncz;
+    const uint32_t cx = cxy / (uint32_t)P.ncy, cy = cxy - cx * (uint32_t)P.ncy;
+    return make_double3(P.xmin + (double)cx * P.cell, P.ymin + (double)cy * P.cell, P.zmin + (double)cz * P.cell);
+}
+
+// A record's position half with w in the fourth component.
+template <class R> __device__ __forceinline__ real4<R> near3(const StepParams& P, double4 pos, R w, uint32_t home);
+template <> __device__ __forceinline__ float4 near3<float>(const StepParams& P, double4 pos, float w, uint32_t home) {
+    const double3 o = cell_origin3(P, home);
+    return make_float4((float)(pos.x - o.x), (float)(pos.y - o.y), (float)(pos.z - o.z), w);
+}
+template <> __device__ __forceinline__ double4 near3<double>(const StepParams&, double4 pos, double w, uint32_t) {
+    return make_double4(pos.x, pos.y, pos.z, w);
+}
+
+// The prediction of PredictHalfStep.cpp:16-22 as reorder3_kernel makes it:
+// (v^, s rho^) and p / rho^2 with p the pressure of the current density.
+template <class R>
+__device__ __forceinline__ real4<R> predict3(const StepParams& P, const real4<R>& va, int itype, const real4<R>& ad,
+                                              R dt, R& pw, R& p) {
+    real4<R> pr;
+    pr.x = va.x + ad.x * dt * R(0.5);
+    pr.y = va.y + ad.y * dt * R(0.5);
+    pr.z = va.z + ad.z * dt * R(0.5);
+    const R rho_hat = P.density_treatment != 0 ? va.w + ad.w * dt * R(0.5) : va.w;
+    p = eos_p<R>(va.w, (R)P.rho0, (R)P.eos_B);
+    pw = p / (rho_hat * rho_hat);
+    pr.w = typed_rho<R>(rho_hat, itype);
+    return pr;
+}
+
+// The reorder of a 3D build step: reorder3_kernel's gather, pressure and
+// cell-start table, plus the records and each slot's home cell.
+template <class R>
+__global__ void __launch_bounds__(kBlock) reorder_list3_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
+                                                               const uint32_t* __restrict__ skey,
+                                                               const uint32_t* __restrict__ perm, StateA3<R> A,
+                                                               StateB3<R> B, ListRec<R>* __restrict__ rec,
+                                                               real4<R>* __restrict__ dens, uint32_t* __restrict__ home,
+                                                               uint32_t* __restrict__ LB,
+                                                               unsigned long long* __restrict__ maxslots) {
+    if (ctrl->halted) return;
+    if (blockIdx.x == 0) {
+        maxslots[threadIdx.x] = 0;
+        maxslots[kMaxSlots + threadIdx.x] = 0;
+    }
+    const uint32_t s0 = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = s0 < P.n;
+    const uint32_t s = live ? s0 : P.n - 1;
+    const uint32_t k = skey[s];
+    const uint32_t src = min(perm[s], P.n - 1);  // a permutation of [0, n); the clamp only guards memory
+    const double4 pos = A.pos[src];
+    const real4<R> va = A.va[src];
+    const real4<R> ad = A.ad[src];
+    const uint32_t oid = A.oid[src];
+    const int itype = (int)pos.w;
+    R pw, p;
+    const real4<R> pr = predict3<R>(P, va, itype, ad, (R)ctrl->cur_dt, pw, p);
+    if (live) {
+        B.pos[s] = pos;
+        B.va[s] = va;
+        B.oid[s] = oid;
+        B.pres[s] = p;
+        ListRec<R> r;
+        r.nr = near3<R>(P, pos, pw, k);
+        r.pr = pr;
+        rec[s] = r;
+        dens[s] = near3<R>(P, pos, typed_rho<R>(va.w, itype), k);
+        home[s] = k;
+    }
+    fill_cell_starts(P, s, live, k, skey, LB);
+}
+
+// The lists of a 3D build step on the reordered copy: for each live
+// particle, every live particle i != s within list_reach, in the order of
+// the reference's cell walk (x-major columns, then y, each column's z range
+// contiguous).  The cells visited are those within list_reach (3 or 4 per
+// axis).
+constexpr uint32_t kCollectL3 = 4;
+__global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl* __restrict__ ctrl,
+                                                             const double4* __restrict__ pos,
+                                                             const uint32_t* __restrict__ skey,
+                                                             const uint32_t* __restrict__ LB,
+                                                             uint32_t* __restrict__ nbl, uint32_t* __restrict__ nbc) {
+    if (ctrl->halted) return;
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    uint32_t cnt = 0;
+    bool bad = false;
+    const double4 pj = pos[s];
+    if ((int)pj.w != 0) {
+        const uint32_t kj = skey[s];
+        const int hz = (int)(kj % (uint32_t)P.ncz);
+        const uint32_t hxy = kj / (uint32_t)P.ncz;
+        const int hx = (int)(hxy / (uint32_t)P.ncy), hy = (int)hxy - hx * P.ncy;
+        // the list pass takes the FP32 positions relative to the home cell
+        // for within 2.1 cells of its origin: a particle outside the cell
+        // grid (keyed to the nearest edge cell) runs the step without lists
+        const double3 o = cell_origin3(P, kj);
+        const double ox = pj.x - o.x, oy = pj.y - o.y, oz = pj.z - o.z;
+        bad = !(ox > -P.cell && ox < 2.0 * P.cell && oy > -P.cell && oy < 2.0 * P.cell && oz > -P.cell &&
+                oz < 2.0 * P.cell);
+        const double reach = P.list_reach * (1.0 + 1e-9);
+        const int x0 = clamped_cell(pj.x - reach, P.xmin, P.cell, P.ncx);
+        const int x1 = clamped_cell(pj.x + reach, P.xmin, P.cell, P.ncx);
+        const int y0 = clamped_cell(pj.y - reach, P.ymin, P.cell, P.ncy);
+        const int y1 = clamped_cell(pj.y + reach, P.ymin, P.cell, P.ncy);
+        const int z0 = clamped_cell(pj.z - reach, P.zmin, P.cell, P.ncz);
+        const int z1 = clamped_cell(pj.z + reach, P.zmin, P.cell, P.ncz);
+#pragma unroll 1
+        for (int x = x0; x <= x1; ++x) {
+#pragma unroll 1
+            for (int y = y0; y <= y1; ++y) {
+                const uint32_t col = ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz;
+                const uint32_t b = LB[col + z0];
+                const uint32_t e = LB[col + z1 + 1];
+                const uint32_t code_xy = (uint32_t)(x - hx + 2) + 5u * (uint32_t)(y - hy + 2);
+                bad |= x - hx < -2 || x - hx > 2 || y - hy < -2 || y - hy > 2;
+#pragma unroll 1
+                for (uint32_t i = b; i < e; i += kCollectL3) {
+                    double4 q[kCollectL3];
+#pragma unroll
+                    for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = pos[min(i + u, e - 1)];
+#pragma unroll
+                    for (uint32_t u = 0; u < kCollectL3; ++u) {
+                        const double dx = q[u].x - pj.x, dy = q[u].y - pj.y, dz = q[u].z - pj.z;
+                        const bool in = (i + u < e) & (i + u != s) & (dx * dx + dy * dy + dz * dz < P.list_r2);
+                        if (in) {
+                            // the z cell of the neighbour (a column's z cells are consecutive keys)
+                            const int dcz = (int)(skey[i + u] - col) - hz;
+                            bad |= dcz < -2 || dcz > 2;
+                            const uint32_t code = code_xy + 25u * (uint32_t)(dcz + 2);
+                            if (cnt < P.list_cap) nbl[(size_t)cnt * P.n + s] = (i + u) | (code << kSlotBits3);
+                            ++cnt;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (cnt > P.list_cap || bad) atomicOr(&ctrl->list_overflow, 1u);
+    nbc[s] = min(cnt, P.list_cap);
+}
+
+// The table of home-cell offsets of the 3D entry codes (first kCodes3
+// threads; a barrier must follow).
+__device__ __forceinline__ void fill_cell_offsets3(const StepParams& P, float4* s_off) {
+    if (threadIdx.x < kCodes3) {
+        const int c = (int)threadIdx.x;
+        const int dx = c % 5 - 2, dy = (c / 5) % 5 - 2, dz = c / 25 - 2;
+        s_off[c] = make_float4((float)(dx * P.cell), (float)(dy * P.cell), (float)(dz * P.cell), 0.f);
+    }
+}
+
+// r_i - r_j of an entry and whether it is within the cut-off (see the
+// header): FP32 from the records, FP64 on the band; FP64 from the FP64
+// record positions.
+__device__ __forceinline__ bool list_cut3(const StepParams& P, const CutFP32& C, const float4* s_off, uint32_t e,
+                                          const float4& ni, const float4& nj, const double4& pj,
+                                          const double4* __restrict__ pos, float& dx, float& dy, float& dz) {
+    const float4 o = s_off[e >> kSlotBits3];
+    dx = (ni.x - nj.x) + o.x;
+    dy = (ni.y - nj.y) + o.y;
+    dz = (ni.z - nj.z) + o.z;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (!(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
+    const double4 q = pos[e & kSlotMask3];  // rare: the band around the cut-off
+    const double ddx = __dsub_rn(q.x, pj.x), ddy = __dsub_rn(q.y, pj.y), ddz = __dsub_rn(q.z, pj.z);
+    dx = (float)ddx;
+    dy = (float)ddy;
+    dz = (float)ddz;
+    return dist2_exact3(ddx, ddy, ddz) < P.cut2;
+}
+__device__ __forceinline__ bool list_cut3(const StepParams& P, const CutFP32&, const float4*, uint32_t,
+                                          const double4& ni, const double4& nj, const double4&,
+                                          const double4* __restrict__, double& dx, double& dy, double& dz) {
+    dx = __dsub_rn(ni.x, nj.x);
+    dy = __dsub_rn(ni.y, nj.y);
+    dz = __dsub_rn(ni.z, nj.z);
+    return dist2_exact3(dx, dy, dz) < P.cut2;
+}
+
+// Summation density from the lists (Density.h:25-61) with the
+// renormalisation of the steps that ask for it (ModelParams.tex:124-133):
+// rho into rec.pr.w (sign-typed), p / rho^2 into rec.nr.w, p into pres.
+template <class R>
+__global__ void __launch_bounds__(kBlock) density_sum_list3(StepParams P, Ctrl* __restrict__ ctrl, ListState3<R> X,
+                                                            const real4<R>* __restrict__ dens,
+                                                            ListRec<R>* __restrict__ rec, R* __restrict__ pres,
+                                                            const uint32_t* __restrict__ nbl,
+                                                            const uint32_t* __restrict__ nbc) {
+    if (ctrl->halted) return;
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    __shared__ float4 s_off[kCodes3];
+    fill_cell_offsets3(P, s_off);
+    __syncthreads();
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    if (s >= P.n) return;
+    const R m = (R)P.mass, inv_h = (R)(1.0 / P.h), w_q1 = (R)P.f_cubic, w_q2 = (R)(P.f_cubic / 6.0);
+    const real4<R> dj = dens[s];
+    const R w0 = w_q1 * (R(2) / R(3));
+    R rho = m * w0;
+    const bool renorm = ctrl->cur_renorm != 0;
+    R wsum = renorm && dj.w != R(0) ? m / fabs(dj.w) * w0 : R(0);
+    bool overflow = false;
+    const uint32_t cnt = dj.w != R(0) ? nbc[s] : 0u;
+    if (cnt) {
+        const CutFP32 C = cut_band(P);
+        const double4 pj = X.pos[s];
+        const uint32_t cap = P.max_nbr - 1;
+        const uint32_t row = P.n * 4u;
+        uint32_t off = s * 4u;
+        uint32_t hits = 0;
+#pragma unroll 2
+        for (uint32_t k = 0; k < cnt; ++k, off += row) {
+            const uint32_t e = list_entry(nbl, off);
+            const real4<R> di = dens[e & kSlotMask3];
+            R dx, dy, dz;
+            const bool hit = list_cut3(P, C, s_off, e, di, dj, pj, X.pos, dx, dy, dz) && di.w != R(0);
+            const bool take = hit & (hits < cap);
+            overflow |= hit & !take;
+            hits += take ? 1u : 0u;
+            const R q = pair_dist<R>(dx, dy, dz) * inv_h;
+            const R t = R(2) - q;
+            const R w = q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q) : w_q2 * t * t * t;
+            const R wt = take ? w : R(0);
+            rho += m * wt;
+            if (renorm) wsum += m / fabs(di.w) * wt;
+        }
+    }
+    if (renorm) rho = rho / wsum;
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+    const int itype = (int)X.pos[s].w;
+    const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);
+    ListRec<R> r = rec[s];
+    r.pr.w = typed_rho<R>(rho, itype);
+    r.nr.w = p / (rho * rho);
+    rec[s] = r;
+    pres[s] = p;
+}
+
+// fast_pair_term3 (the cubic configuration) for one list entry, without
+// branches except the rare Lennard-Jones case; nothing is added when take
+// is false.  i's records: nr (position half, p / rho^2), pr (v^, s rho^),
+// vai (its current velocity; continuity only).
+template <class R>
+__device__ __forceinline__ void list_pair_term3(const FastPair<R>& K, R (&S)[12], bool take, R dx, R dy, R dz,
+                                                const real4<R>& nri, const real4<R>& pri, const real4<R>& vai,
+                                                const real4<R>& vaj, const real4<R>& prj, R pwj, int tj) {
+    // S: sx, sy, sz, sd, xx, xy, xz, amu, ljx, ljy, ljz, (unused)
+    const R d2 = dx * dx + dy * dy + dz * dz;
+    const R rinv = d2 > R(0) ? fast_rsqrt<R>(d2) : R(0);
+    const R dist = d2 * rinv;
+    const R q = dist * K.inv_h;
+    const R t = R(2) - q;
+    const R t2 = t * t;
+    const bool inner = q <= R(1);
+    const R F1 = K.c_q1 * (R(1.5) * q - R(2));
+    const R F2 = -K.c_q2 * t2 * rinv;
+    const R W1 = K.w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q);
+    const R W2 = K.w_q2 * t2 * t;
+    const R F = take ? (inner ? F1 : F2) : R(0);
+    const R W = inner ? W1 : W2;
+    if (K.cont) S[3] += F * ((vai.x - vaj.x) * dx + (vai.y - vaj.y) * dy + (vai.z - vaj.z) * dz);
+    if (K.lj_on && take && tj > 0 && pri.w < R(0) && dist < K.lj_r0) {
+        const R rn = K.lj_r0 * rinv;
+        const R rn2 = rn * rn, rn4 = rn2 * rn2;
+        const R f = K.lj_D * (rn4 * rn4 * rn4 - rn4) * (rinv * rinv);
+        S[8] -= dx * f;
+        S[9] -= dy * f;
+        S[10] -= dz * f;
+    }
+    const R inv_rs = fast_rcp<R>(fabs(pri.w) + fabs(prj.w));
+    R coef = nri.w + pwj;
+    const R dvx = pri.x - prj.x, dvy = pri.y - prj.y, dvz = pri.z - prj.z;
+    if (K.av_on) {
+        const R vr = dvx * dx + dvy * dy + dvz * dz;
+        const R mu = K.h * vr * fast_rcp<R>(d2 + K.eta2);
+        const bool app = take & (vr < R(0));
+        S[7] = app ? fmax(S[7], -mu) : S[7];
+        coef += app ? (K.beta * mu - K.alpha_c) * mu * R(2) * inv_rs : R(0);
+    }
+    const R fc = F * coef;
+    S[0] += fc * dx;
+    S[1] += fc * dy;
+    S[2] += fc * dz;
+    if (K.xsph_on) {
+        const R cw = (take & (pri.w > R(0))) ? W * inv_rs : R(0);
+        S[4] += dvx * cw;
+        S[5] += dvy * cw;
+        S[6] += dvz * cw;
+    }
+}
+
+// Pair pass from the 3D lists: the cubic configuration of pair3_kernel
+// (continuity, pressure, artificial viscosity, Lennard-Jones walls, XSPH),
+// the dt maxima and the fields for the integrator.  Entries in groups of
+// G, as for_each_listed.
+template <class R>
+__global__ void __launch_bounds__(kBlock) pair_list3_kernel(StepParams P, Ctrl* __restrict__ ctrl, ListState3<R> X,
+                                                            const ListRec<R>* __restrict__ rec, R* __restrict__ pres,
+                                                            const uint32_t* __restrict__ nbl,
+                                                            const uint32_t* __restrict__ nbc,
+                                                            real4<R>* __restrict__ der, real4<R>* __restrict__ xs,
+                                                            unsigned long long* __restrict__ maxslots) {
+    if (ctrl->halted) return;
+    if (ctrl->list_overflow) {  // the build of this step ran out of room: run it without lists
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            ctrl->halted = 1;
+            ctrl->status = kStatusListOverflow;
+        }
+        return;
+    }
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    __shared__ float4 s_off[kCodes3];
+    fill_cell_offsets3(P, s_off);
+    __syncthreads();
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    const bool live = s < P.n;
+    const FastPair<R> K(P);
+    R S[12] = {};
+    bool overflow = false;
+    double4 pj = make_double4(0, 0, 0, 0);
+    real4<R> vaj{};
+    ListRec<R> rj{};
+    if (live) {
+        pj = X.pos[s];
+        vaj = X.va[s];
+        rj = rec[s];
+    }
+    const int tj = (int)pj.w;
+    const uint32_t cnt = live && tj != 0 ? nbc[s] : 0u;
+    const uint32_t cap = P.max_nbr - 1;
+    if (cnt) {
+        constexpr uint32_t G = sizeof(R) == 4 ? 4 : 2;
+        static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
+        const bool capped = __ballot(cnt > cap) != 0;  // some list of the wave may hold more than cap neighbours
+        const CutFP32 C = cut_band(P);
+        const uint32_t row = P.n * 4u;
+        uint32_t off = s * 4u;
+        uint32_t hits = 0;
+        bool over = false;
+        uint32_t e[G];
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, off + u * row);
+        off += G * row;
+#pragma unroll 1
+        for (uint32_t k = 0; k < cnt; k += G) {
+            ListRec<R> r[G];
+            real4<R> va[G]{};
+            uint32_t en[G];
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                r[u] = rec[e[u] & kSlotMask3];
+                if (K.cont) va[u] = X.va[e[u] & kSlotMask3];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, off + u * row);
+            off += G * row;
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                R dx, dy, dz;
+                const bool in = list_cut3(P, C, s_off, e[u], r[u].nr, rj.nr, pj, X.pos, dx, dy, dz);
+                const bool hit = in & (r[u].pr.w != R(0)) & (k + u < cnt);
+                bool take = hit;
+                if (capped) {
+                    take = hit & (hits < cap);
+                    over |= hit & !take;
+                    hits += take ? 1u : 0u;
+                }
+                list_pair_term3<R>(K, S, take, dx, dy, dz, r[u].nr, r[u].pr, va[u], vaj, rj.pr, rj.nr.w, tj);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) e[u] = en[u];
+        }
+        overflow |= over;
+    }
+    if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
+    const R m = (R)P.mass;
+    const R ax = S[8] + m * S[0];
+    const R ay = -(R)P.g + S[9] + m * S[1];
+    const R az = S[10] + m * S[2];
+    const R drho = m * S[3];
+    const R xf = m * R(2) * (R)P.xsph_eps;
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1)
+        block_max2_to_slots(live ? sqrt((double)ax * ax + (double)ay * ay + (double)az * az) : 0.0,
+                            live ? (double)S[7] : 0.0, maxslots, lb);
+    if (live) {
+        real4<R> d;
+        d.x = ax;
+        d.y = ay;
+        d.z = az;
+        d.w = drho;
+        der[s] = d;
+        real4<R> x;
+        x.x = xf * S[4];
+        x.y = xf * S[5];
+        x.z = xf * S[6];
+        x.w = S[7];
+        xs[s] = x;
+        if (P.density_treatment != 0) pres[s] = eos_p<R>(vaj.w, (R)P.rho0, (R)P.eos_B);
+    }
+}
+
+// whole_step of every slot from X into A (in place on list steps), the
+// cell key for the next build, the step's largest displacement, and the
+// next step's records.
+template <class R>
+__global__ void __launch_bounds__(kBlock) integrate_list3_kernel(StepParams P, Ctrl* __restrict__ ctrl,
+                                                                 ListState3<R> X, const real4<R>* __restrict__ der,
+                                                                 const real4<R>* __restrict__ xs,
+                                                                 ListRec<R>* __restrict__ rec,
+                                                                 real4<R>* __restrict__ dens,
+                                                                 const uint32_t* __restrict__ home, StateA3<R> A,
+                                                                 uint32_t* __restrict__ key_next,
+                                                                 const unsigned long long* __restrict__ maxslots,
+                                                                 unsigned long long* __restrict__ dslots) {
+    Ctrl& c = *ctrl;
+    if (c.halted) return;
+    double dt;
+    if (P.dt_method == 2 && P.dt_dynamic_mode == 1) {
+        double amax, mumax;
+        slots_max2(maxslots, amax, mumax);
+        dt = P.cfl * fmin(sqrt(P.h / amax), P.h / (P.c0 + mumax));
+    } else {
+        dt = c.cur_dt_const;
+    }
+    const bool never = P.dt_method == 2 && dt <= P.never_ending;
+    const bool nbr_err = P.consistency_check && c.nbr_overflow;
+    if (never || nbr_err) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            c.halted = 1;
+            c.status = nbr_err ? 3u : 4u;
+        }
+        return;
+    }
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t outside = 0, bad = 0, changed = 0;
+    double moved = 0.0;
+    if (s < P.n) {
+        const real4<R> d = der[s];
+        const real4<R> x = xs[s];
+        const double4 p0 = X.pos[s];
+        real4<R> pr{};
+        if (P.density_treatment == 0) pr.w = fabs(rec[s].pr.w);  // the summed density
+        const uint32_t h = home[s];
+        const Stepped3<R> o = integrate_one3<R>(P, c, s, p0, X.va[s], pr, X.oid[s], d.x, d.y, d.z, d.w, x.x, x.y,
+                                                x.z, dt, A, key_next, h, outside, bad, changed);
+        const double mx = o.pos.x - p0.x, my = o.pos.y - p0.y, mz = o.pos.z - p0.z;
+        moved = sqrt(mx * mx + my * my + mz * mz);
+        const int itype = (int)o.pos.w;
+        R pw, p;
+        ListRec<R> r;
+        r.pr = predict3<R>(P, o.va, itype, d, (R)dt, pw, p);
+        r.nr = near3<R>(P, o.pos, pw, h);
+        rec[s] = r;
+        dens[s] = near3<R>(P, o.pos, typed_rho<R>(o.va.w, itype), h);
+    }
+    wave_count_add(&c.outside, outside);
+    if (bad) atomicOr(&c.grid_overflow, 1u);
+    block_max2_to_slots(moved, 0.0, dslots, blockIdx.x);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.dt_new = dt;
+        c.pending = 1;
+    }
+}
+
+}  // namespace rrsph

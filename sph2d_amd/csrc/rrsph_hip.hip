@@ -25,6 +25,7 @@
 #include "step3d.hpp"
 #include "pair_tile.hpp"
 #include "list_step.hpp"
+#include "list_step3.hpp"
 
 using namespace rrsph;
 
@@ -203,6 +204,7 @@ struct rrsph_ctx {
     bool list_suspended = false;        // the adjoint's tapes and replays run without lists
     uint32_t *nbl = nullptr, *nbc = nullptr;
     void* rec = nullptr;                // records of the list pass (ListRec per slot)
+    void* dens = nullptr;               // 3D: records of the summation density pass (real4 per slot)
     uint32_t* home = nullptr;           // each slot's cell at the last build
     unsigned long long* dslots = nullptr;  // per-step largest displacement (block maxima)
     hipGraph_t graph_list = nullptr, graph_build = nullptr;
@@ -457,7 +459,55 @@ template <class R> void enqueue_list_step_t(rrsph_ctx* c, Profiler* prof, bool b
     });
 }
 
+// One 3D neighbour-list step (build: sort, reorder and new lists first).
+template <class R> void enqueue_list_step3_t(rrsph_ctx* c, Profiler* prof, bool build) {
+    Launcher L{c, prof};
+    const StepParams& P = c->P;
+    const uint32_t nb = grid_blocks(P.n, kBlock);
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;
+    L(K_PROLOGUE, [&] {
+        list_prologue<<<1, kBlock, 0, c->stream>>>(P, c->ctrl, c->dslots, c->maxslots, build ? 1u : 0u);
+    });
+    ListState3<R> X{(const double4*)c->posA, (const real4<R>*)c->vaA, c->oidA};
+    ListRec<R>* rec = (ListRec<R>*)c->rec;
+    real4<R>* dens = (real4<R>*)c->dens;
+    if (build) {
+        enqueue_sort(c, c->ctrl, L);
+        StateA3<R> A = stateA3<R>(c);
+        StateB3<R> B = stateB3<R>(c, c->pred, c->pw, c->pres);
+        L(K_REORDER, [&] {
+            reorder_list3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B,
+                                                                  rec, dens, c->home, c->LB, c->maxslots);
+        });
+        L(K_LIST_BUILD, [&] {
+            list_build3_kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, (const double4*)c->posB, c->sorted_keys,
+                                                                 c->LB, c->nbl, c->nbc);
+        });
+        X = ListState3<R>{(const double4*)c->posB, (const real4<R>*)c->vaB, c->oidB};
+    }
+    if (P.density_treatment == 0) {
+        L(K_DENSITY_SUM, [&] {
+            density_sum_list3<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, dens, rec, (R*)c->pres, c->nbl,
+                                                                   c->nbc);
+        });
+    }
+    L(K_PAIR, [&] {
+        pair_list3_kernel<R><<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, X, rec, (R*)c->pres, c->nbl, c->nbc,
+                                                               (real4<R>*)c->der, (real4<R>*)c->xs, c->maxslots);
+    });
+    L(K_INTEGRATE, [&] {
+        integrate_list3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, X, (const real4<R>*)c->der,
+                                                                (const real4<R>*)c->xs, rec, dens, c->home,
+                                                                stateA3<R>(c), c->key_next, c->maxslots, c->dslots);
+    });
+}
+
 void enqueue_list_step(rrsph_ctx* c, Profiler* prof, bool build) {
+    if (c->P.dim == 3) {
+        if (c->fp32) enqueue_list_step3_t<float>(c, prof, build);
+        else enqueue_list_step3_t<double>(c, prof, build);
+        return;
+    }
     if (c->fp32) enqueue_list_step_t<float>(c, prof, build);
     else enqueue_list_step_t<double>(c, prof, build);
 }
@@ -798,18 +848,26 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
     ctx->fast_pair = p.density_skf == 1 && p.intf_skf == 1 && p.artificial_viscosity_skf == 1 &&
                      p.average_velocity_skf == 1 && p.intf_sph_approximation == 2 && !p.artificial_pressure &&
                      (p.density_treatment == 0 || p.density_treatment == 1) && std::fabs(p.cell_scale_k - 2.0) == 0.0;
-    // neighbour-list steps: the 2D cubic configuration; skin as a fraction of the cut-off
+    // neighbour-list steps: the cubic configuration, 2D and 3D; skin as a
+    // fraction of the cut-off (an entry holds a slot in kSlotBits bits in
+    // 2D, kSlotBits3 in 3D)
     double skin_frac = 0.125;
     if (const char* e = std::getenv("RRSPH_HIP_LIST_SKIN")) skin_frac = std::strtod(e, nullptr);
-    // (an entry holds a slot in kSlotBits bits)
-    ctx->list_mode = !d3 && ctx->fast_pair && skin_frac > 0.0 && skin_frac < 0.9 && P.n <= kSlotMask;
+    ctx->list_mode = ctx->fast_pair && skin_frac > 0.0 && skin_frac < 0.9 && P.n <= (d3 ? kSlotMask3 : kSlotMask);
     if (const char* e = std::getenv("RRSPH_HIP_LIST")) ctx->list_mode = ctx->list_mode && std::strcmp(e, "0") != 0;
     P.list_reach = P.cell * (1.0 + skin_frac);
     P.list_r2 = P.list_reach * P.list_reach * (1.0 + 1e-12);
     P.skin_half = 0.5 * P.cell * skin_frac * (1.0 - 1e-9);
-    P.list_cap = (uint32_t)std::ceil(p.max_neighbours * (1.0 + skin_frac) * (1.0 + skin_frac)) + 8;
-    // the list pass addresses the lists (list_cap + kListPad rows) with 32-bit byte offsets
-    if ((uint64_t)(P.list_cap + kListPad) * P.n * 4u > 0xffffffffull) ctx->list_mode = false;
+    const double grow = std::pow(1.0 + skin_frac, d3 ? 3 : 2);
+    P.list_cap = (uint32_t)std::ceil(p.max_neighbours * grow) + 8;
+    // the list pass addresses the lists (list_cap + kListPad rows) with
+    // 32-bit byte offsets: fewer rows where n is large (a build that finds
+    // more entries for a particle runs its step without lists)
+    const uint64_t rows_max = 0xffffffffull / (4ull * P.n);
+    if (P.list_cap + kListPad > rows_max) {
+        if (rows_max >= (uint64_t)p.max_neighbours + kListPad) P.list_cap = (uint32_t)(rows_max - kListPad);
+        else ctx->list_mode = false;
+    }
     if (d3 && !ctx->fast_pair)
         return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
                     "RRSPH_HIP: 3D supports cubic kernels for every term, approximation 2, no artificial pressure, "
@@ -837,7 +895,7 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->rec, c->home, c->dslots};
+                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->rec, c->dens, c->home, c->dslots};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -956,6 +1014,7 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
         ctx->nbl = dalloc<uint32_t>((size_t)(ctx->P.list_cap + kListPad) * n, e);
         ctx->nbc = dalloc<uint32_t>(n, e);
         ctx->rec = dalloc<ListRec<double>>(n, e);
+        if (dim == 3) ctx->dens = dalloc<double4>(n, e);
         ctx->home = dalloc<uint32_t>(n, e);
         ctx->dslots = dalloc<unsigned long long>(2 * kMaxSlots, e);
     }

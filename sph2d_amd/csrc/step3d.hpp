@@ -253,8 +253,14 @@ __global__ void __launch_bounds__(kBlock) density_sum3_kernel(StepParams P, Ctrl
 }
 
 // ------------------------------------------------------------------ whole_step
+// whole_step of slot s into A (WholeStep.cpp:19-61 with boundary motion);
+// returns the new position (with the type in w) and velocity/density.
+template <class R> struct Stepped3 {
+    double4 pos;
+    real4<R> va;
+};
 template <class R>
-__device__ __forceinline__ void integrate_one3(const StepParams& P, const Ctrl& c, uint32_t s, double4 pos,
+__device__ __forceinline__ Stepped3<R> integrate_one3(const StepParams& P, const Ctrl& c, uint32_t s, double4 pos,
                                                real4<R> va, const real4<R>& pred, uint32_t oid, R ax, R ay, R az,
                                                R drho, R avx, R avy, R avz, double dt, StateA3<R> A,
                                                uint32_t* __restrict__ key_next, uint32_t key_old, uint32_t& outside,
@@ -320,6 +326,7 @@ __device__ __forceinline__ void integrate_one3(const StepParams& P, const Ctrl& 
     const uint32_t key = itype == 0 ? P.ncells : cell_key3(P, pos.x, pos.y, pos.z, bad);
     key_next[s] = key;
     changed = key != key_old ? 1u : 0u;
+    return Stepped3<R>{pos, nva};
 }
 
 // ------------------------------------------------------------------ pair pass

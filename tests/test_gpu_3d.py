@@ -119,6 +119,7 @@ def test_3d_fp32_error_vs_fp64_oracle():
 def test_3d_incremental_resort_equals_full_sort(monkeypatch):
     lay = G.dam_break_3d()
     prm = lay.params
+    monkeypatch.setenv("RRSPH_HIP_LIST", "0")  # the sorted step of every step
     out = {}
     for mode in ("full", "incremental"):
         if mode == "full":

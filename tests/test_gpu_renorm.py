@@ -1,7 +1,7 @@
 """The renormalised summation density (the boundary-deficiency correction of
 docs/latex/ModelParams.tex:124-133, density_normalization = 1) on the GPU
 against the FP64 oracle: rho_j = sum m W / sum (m / rho_i) W on every n-th
-step, plain summation in between; 2D (list and sorted steps) and 3D."""
+step, plain summation in between; 2D and 3D, list and sorted steps."""
 import copy
 
 import numpy as np
@@ -47,9 +47,11 @@ def test_renormalised_summation_2d(monkeypatch, lists, precision):
         assert rho_err < 1e-5 and np.max(np.abs(g["r"] - r["r"])) < 1e-6 * prm.delta
 
 
+@pytest.mark.parametrize("lists", ["1", "0"])
 @pytest.mark.parametrize("precision", [H.FP64, H.FP32])
-def test_renormalised_summation_3d(precision):
+def test_renormalised_summation_3d(monkeypatch, lists, precision):
     """The small 3D dam break (delta 0.04 m), renormalised every 2nd step, 6 steps."""
+    monkeypatch.setenv("RRSPH_HIP_LIST", lists)
     lay = G.dam_break_3d(delta=0.04)
     prm = copy.copy(lay.params)
     prm.density_normalization = 1
