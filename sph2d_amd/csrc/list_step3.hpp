@@ -118,26 +118,50 @@ __global__ void __launch_bounds__(kBlock) reorder_list3_kernel(StepParams P, con
 // particle, every live particle i != s within list_reach, in the order of
 // the reference's cell walk (x-major columns, then y, each column's z range
 // contiguous).  The cells visited are those within list_reach (3 or 4 per
-// axis).
+// axis).  The walk is wave-uniform: the slots of a wave share one (x, y)
+// column of cells or a few, so the wave loops over the union of its lanes'
+// columns and, in each, over its longest z range, lanes masked outside
+// their own ranges (same entries and order as a per-lane walk).  Entries
+// go to the lane's column of an LDS buffer first; when a lane's column
+// might overflow, the wave writes the buffered rows out together, so each
+// row's stores from the lanes of a wave are contiguous (written straight
+// from the walk, they left lines partly written in L2).
 constexpr uint32_t kCollectL3 = 4;
+constexpr uint32_t kBuildChunk = 32;  // LDS rows per lane
+__device__ __forceinline__ int wave_min_i32(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
 __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl* __restrict__ ctrl,
                                                              const double4* __restrict__ pos,
                                                              const uint32_t* __restrict__ skey,
                                                              const uint32_t* __restrict__ LB,
                                                              uint32_t* __restrict__ nbl, uint32_t* __restrict__ nbc) {
+    __shared__ uint32_t buf[kBuildChunk * kBlock];
     if (ctrl->halted) return;
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
-    const uint32_t s = lb * kBlock + threadIdx.x;
-    if (s >= P.n) return;
+    const uint32_t s0 = lb * kBlock + threadIdx.x;
+    const bool inb = s0 < P.n;
+    const uint32_t s = inb ? s0 : P.n - 1;
     uint32_t cnt = 0;
     bool bad = false;
     const double4 pj = pos[s];
-    if ((int)pj.w != 0) {
-        const uint32_t kj = skey[s];
-        const int hz = (int)(kj % (uint32_t)P.ncz);
-        const uint32_t hxy = kj / (uint32_t)P.ncz;
-        const int hx = (int)(hxy / (uint32_t)P.ncy), hy = (int)hxy - hx * P.ncy;
+    const bool alive = inb && (int)pj.w != 0;
+    const uint32_t kj = skey[s];
+    const int hz = (int)(kj % (uint32_t)P.ncz);
+    const uint32_t hxy = kj / (uint32_t)P.ncz;
+    const int hx = (int)(hxy / (uint32_t)P.ncy), hy = (int)hxy - hx * P.ncy;
+    const double reach = P.list_reach * (1.0 + 1e-9);
+    // empty ranges (x0 > x1) for lanes with nothing to list
+    int x0 = 1, x1 = 0, y0 = 1, y1 = 0, z0 = 0, z1 = 0;
+    if (alive) {
         // the list pass takes the FP32 positions relative to the home cell
         // for within 2.1 cells of its origin: a particle outside the cell
         // grid (keyed to the nearest edge cell) runs the step without lists
@@ -145,44 +169,58 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
         const double ox = pj.x - o.x, oy = pj.y - o.y, oz = pj.z - o.z;
         bad = !(ox > -P.cell && ox < 2.0 * P.cell && oy > -P.cell && oy < 2.0 * P.cell && oz > -P.cell &&
                 oz < 2.0 * P.cell);
-        const double reach = P.list_reach * (1.0 + 1e-9);
-        const int x0 = clamped_cell(pj.x - reach, P.xmin, P.cell, P.ncx);
-        const int x1 = clamped_cell(pj.x + reach, P.xmin, P.cell, P.ncx);
-        const int y0 = clamped_cell(pj.y - reach, P.ymin, P.cell, P.ncy);
-        const int y1 = clamped_cell(pj.y + reach, P.ymin, P.cell, P.ncy);
-        const int z0 = clamped_cell(pj.z - reach, P.zmin, P.cell, P.ncz);
-        const int z1 = clamped_cell(pj.z + reach, P.zmin, P.cell, P.ncz);
+        x0 = clamped_cell(pj.x - reach, P.xmin, P.cell, P.ncx);
+        x1 = clamped_cell(pj.x + reach, P.xmin, P.cell, P.ncx);
+        y0 = clamped_cell(pj.y - reach, P.ymin, P.cell, P.ncy);
+        y1 = clamped_cell(pj.y + reach, P.ymin, P.cell, P.ncy);
+        z0 = clamped_cell(pj.z - reach, P.zmin, P.cell, P.ncz);
+        z1 = clamped_cell(pj.z + reach, P.zmin, P.cell, P.ncz);
+        bad |= x0 < hx - 2 || x1 > hx + 2 || y0 < hy - 2 || y1 > hy + 2 || z0 < hz - 2 || z1 > hz + 2;
+    }
+    uint32_t* my = buf + threadIdx.x;
+    uint32_t nh = 0;
+    auto flush = [&]() {
 #pragma unroll 1
-        for (int x = x0; x <= x1; ++x) {
+        for (uint32_t r = 0; r < nh; ++r)
+            if (cnt + r < P.list_cap) nbl[(size_t)(cnt + r) * P.n + s] = my[r * kBlock];
+        cnt += nh;
+        nh = 0;
+    };
+    const int wx0 = wave_min_i32(alive ? x0 : INT_MAX), wx1 = wave_max_i32(alive ? x1 : INT_MIN);
+    const int wy0 = wave_min_i32(alive ? y0 : INT_MAX), wy1 = wave_max_i32(alive ? y1 : INT_MIN);
 #pragma unroll 1
-            for (int y = y0; y <= y1; ++y) {
-                const uint32_t col = ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz;
-                const uint32_t b = LB[col + z0];
-                const uint32_t e = LB[col + z1 + 1];
-                const uint32_t code_xy = (uint32_t)(x - hx + 2) + 5u * (uint32_t)(y - hy + 2);
-                bad |= x - hx < -2 || x - hx > 2 || y - hy < -2 || y - hy > 2;
+    for (int x = wx0; x <= wx1; ++x) {
 #pragma unroll 1
-                for (uint32_t i = b; i < e; i += kCollectL3) {
-                    double4 q[kCollectL3];
+        for (int y = wy0; y <= wy1; ++y) {
+            const bool need = x >= x0 && x <= x1 && y >= y0 && y <= y1;
+            const uint32_t col = need ? ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz : 0u;
+            const uint32_t b = need ? LB[col + z0] : 0u;
+            const uint32_t len = need ? LB[col + z1 + 1] - b : 0u;
+            const uint32_t code_xy = (uint32_t)(x - hx + 2) + 5u * (uint32_t)(y - hy + 2);
+            const uint32_t lmax = wave_max_u32(len);
+#pragma unroll 1
+            for (uint32_t i = 0; i < lmax; i += kCollectL3) {
+                double4 q[kCollectL3];
 #pragma unroll
-                    for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = pos[min(i + u, e - 1)];
+                for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = pos[len ? b + min(i + u, len - 1) : s];
 #pragma unroll
-                    for (uint32_t u = 0; u < kCollectL3; ++u) {
-                        const double dx = q[u].x - pj.x, dy = q[u].y - pj.y, dz = q[u].z - pj.z;
-                        const bool in = (i + u < e) & (i + u != s) & (dx * dx + dy * dy + dz * dz < P.list_r2);
-                        if (in) {
-                            // the z cell of the neighbour (a column's z cells are consecutive keys)
-                            const int dcz = (int)(skey[i + u] - col) - hz;
-                            bad |= dcz < -2 || dcz > 2;
-                            const uint32_t code = code_xy + 25u * (uint32_t)(dcz + 2);
-                            if (cnt < P.list_cap) nbl[(size_t)cnt * P.n + s] = (i + u) | (code << kSlotBits3);
-                            ++cnt;
-                        }
+                for (uint32_t u = 0; u < kCollectL3; ++u) {
+                    const double dx = q[u].x - pj.x, dy = q[u].y - pj.y, dz = q[u].z - pj.z;
+                    const bool in = (i + u < len) & (b + i + u != s) & (dx * dx + dy * dy + dz * dz < P.list_r2);
+                    if (in) {
+                        // the z cell of the neighbour (a column's z cells are consecutive keys)
+                        const int dcz = (int)(skey[b + i + u] - col) - hz;
+                        const uint32_t code = code_xy + 25u * (uint32_t)(dcz + 2);
+                        my[nh * kBlock] = (b + i + u) | (code << kSlotBits3);
+                        ++nh;
                     }
                 }
+                if (__any(nh + kCollectL3 > kBuildChunk)) flush();  // wave-uniform
             }
         }
     }
+    flush();
+    if (!inb) return;
     if (cnt > P.list_cap || bad) atomicOr(&ctrl->list_overflow, 1u);
     nbc[s] = min(cnt, P.list_cap);
 }
