@@ -328,14 +328,16 @@ __device__ __forceinline__ uint32_t list_entry(const uint32_t* __restrict__ nbl,
 // count.  Entries go in groups of G: the group's records are gathered
 // together and the next group's entries loaded behind them, so that no
 // gathered value lives across iterations (the loop waits only on its own
-// loads) and 2 G gathers are in flight at the first entry's use.
+// loads).  Larger groups keep more gathers in flight per wave but take
+// registers: the pass is bound by load latency, and 2 entries per group
+// (74 VGPRs, 6 waves per SIMD) gave the least time.
 template <class R, bool kCap, class F>
 __device__ __forceinline__ void for_each_listed(const StepParams& P, const float2* s_off, uint32_t cap, uint32_t s,
                                                 const real4<R>& nj, double2 pj, const double2* __restrict__ pos,
                                                 const ListRec<R>* __restrict__ rec, const uint32_t* __restrict__ nbl,
                                                 uint32_t cnt, bool& overflow,
                                                 F&& f) {
-    constexpr uint32_t G = sizeof(R) == 4 ? 4 : 2;
+    constexpr uint32_t G = 2;  // measured at C1-1M FP32: 57.7 us per pass against 58.8 (3), 61.4 (4), 65.1 (1)
     static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
     if (cnt == 0) return;
     const uint32_t row = P.n * 4u;
