@@ -13,6 +13,7 @@
 #pragma once
 
 #include "adjoint.hpp"
+#include "list_step3.hpp"
 #include "step3d.hpp"
 
 namespace rrsph {
@@ -243,11 +244,15 @@ __device__ __forceinline__ void adj_orient3(const AdjConsts& K, const AdjPair& Q
 
 // Gather over neighbours, both orientations of every pair, then the
 // predict and EOS reverse; writes lam_out.  Every lane of a block runs the
-// wave-uniform walk (lanes past n walk empty ranges).
-template <class R>
+// wave-uniform walk (lanes past n walk empty ranges).  LISTS: the
+// neighbours come from the lists of the build the step read (nbl, nbc, in
+// B's slot order) instead of the cell walk.
+template <class R, bool LISTS>
 __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, StateB3<R> B,
                                                     const uint32_t* __restrict__ skey,
                                                     const uint32_t* __restrict__ LB,
+                                                    const uint32_t* __restrict__ nbl,
+                                                    const uint32_t* __restrict__ nbc,
                                                     const double4* __restrict__ out_a,
                                                     const double4* __restrict__ out_x,
                                                     const double* __restrict__ direct, double* __restrict__ lam_out,
@@ -373,8 +378,11 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
             Pb += coefb;
         }
     };
-    collect_neighbours3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, skey[sc], pk, B.pos, LB, my, overflow,
-                                   flush);
+    if (LISTS)
+        collect_listed3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, pk, B.pos, nbl, nbc, my, overflow, flush);
+    else
+        collect_neighbours3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, skey[sc], pk, B.pos, LB, my,
+                                       overflow, flush);
     if (overflow) atomicOr(overflow_flag, 1u);
     if (!live) return;
     // P = p / R^2, p = EOS(rho) = B ((rho / rho0)^7 - 1), u = v + a_prev dt/2, R = rho + drho_prev dt/2

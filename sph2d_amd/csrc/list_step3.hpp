@@ -225,6 +225,51 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
     nbc[s] = min(cnt, P.list_cap);
 }
 
+// collect_neighbours3 over a particle's list instead of the cell walk: the
+// entries within the cut-off (the reference's FP64 test on pos), alive, in
+// list order, first `cap` kept, into the lane's column of the LDS hit list,
+// flushed wave-uniformly.  Every lane of the wave calls it; lanes with
+// active = false have no entries.
+template <uint32_t kChunk, class F>
+__device__ __forceinline__ void collect_listed3(const StepParams& P, bool active, uint32_t cap, uint32_t s,
+                                                double4 pj, const double4* __restrict__ pos,
+                                                const uint32_t* __restrict__ nbl, const uint32_t* __restrict__ nbc,
+                                                uint32_t* __restrict__ hits, bool& overflow, F&& flush) {
+    static_assert(kChunk >= 2 * kCollectL3, "hit list too short");
+    const uint32_t cnt = active ? nbc[s] : 0u;
+    const uint32_t cmax = wave_max_u32(cnt);
+    const uint32_t row = P.n * 4u;
+    uint32_t nh = 0, done = 0;
+#pragma unroll 1
+    for (uint32_t k = 0; k < cmax; k += kCollectL3) {
+        uint32_t i[kCollectL3];
+        double4 q[kCollectL3];
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) i[u] = list_entry(nbl, (k + u) * row + s * 4u) & kSlotMask3;
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = pos[i[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) {
+            const double dx = __dsub_rn(q[u].x, pj.x), dy = __dsub_rn(q[u].y, pj.y), dz = __dsub_rn(q[u].z, pj.z);
+            const bool hit = (k + u < cnt) & ((int)q[u].w != 0) & (dist2_exact3(dx, dy, dz) < P.cut2);
+            if (hit) {
+                if (done + nh < cap) {
+                    hits[nh * kBlock] = i[u];
+                    ++nh;
+                } else {
+                    overflow = true;
+                }
+            }
+        }
+        if (__any(nh + kCollectL3 > kChunk)) {  // wave-uniform
+            flush(nh);
+            done += nh;
+            nh = 0;
+        }
+    }
+    flush(nh);
+}
+
 // The table of home-cell offsets of the 3D entry codes (first kCodes3
 // threads; a barrier must follow).
 __device__ __forceinline__ void fill_cell_offsets3(const StepParams& P, float4* s_off) {

@@ -131,6 +131,22 @@ struct AdjTape {
     double *lam[2] = {nullptr, nullptr}, *direct = nullptr, *drho_adj = nullptr;
     double4* out_adj = nullptr;          // 3D: out_a (n) then out_x (n)
     uint32_t* flag = nullptr;
+    // 3D with neighbour lists: the tape runs list steps.  The forward builds
+    // at every checkpoint step and records which steps built; a replay
+    // repeats that schedule, so it repeats its steps exactly.  The segment
+    // in hand keeps the lists (and sorted cell keys) of each of its builds;
+    // the reverse step of a list step reads its state in the slot order of
+    // its build (identity reorder) and that build's lists.
+    bool lists = false;
+    std::vector<uint8_t> builds;         // per step: the step built lists
+    struct Lists {
+        uint32_t *nbl = nullptr, *nbc = nullptr, *keys = nullptr;
+    };
+    std::vector<Lists> seg_lists;        // lists of the segment's builds (allocated on demand, reused)
+    uint32_t nseg_lists = 0;             // builds of the segment in hand
+    std::vector<int32_t> seg_build;      // per seg slot: the build (index in seg_lists) its step read
+    uint32_t* iota = nullptr;            // identity permutation
+    uint64_t force_build = ~0ull;        // tape step that must build (after a stale-list halt)
 };
 
 struct rrsph_ctx {
@@ -879,6 +895,13 @@ void free_tape(AdjTape& T) {
     for (void* p : T.ck)
         if (p) (void)hipFree(p);
     T.ck.clear();
+    for (auto& L : T.seg_lists) {
+        void* lp[] = {L.nbl, L.nbc, L.keys};
+        for (void* p : lp)
+            if (p) (void)hipFree(p);
+    }
+    T.seg_lists.clear();
+    if (T.iota) (void)hipFree(T.iota);
     void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
                     T.drho_adj, T.out_adj, T.flag, T.rec, T.seg_keys, T.seg_perm};
     for (void* p : ptrs)
@@ -922,7 +945,8 @@ void enqueue_state_keys(rrsph_ctx* ctx) {
 // arithmetic FP64): whole_step reverse, then the pair gather.
 template <class R>
 static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double* lam_in, double* lam_out,
-                                 double* psum, const uint32_t* keys, const uint32_t* perm) {
+                                 double* psum, const uint32_t* keys, const uint32_t* perm,
+                                 const AdjTape::Lists* lists) {
     AdjTape& T = ctx->adj;
     const StepParams& P = ctx->P;
     const size_t n = P.n;
@@ -937,8 +961,14 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     if (P.dim == 3) {
         StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
         adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.out_adj + n, T.direct, psum);
-        adj_pair3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.out_adj + n,
-                                                     T.direct, lam_out, T.flag);
+        if (lists)
+            adj_pair3<R, true><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, lists->nbl,
+                                                               lists->nbc, T.out_adj, T.out_adj + n, T.direct,
+                                                               lam_out, T.flag);
+        else
+            adj_pair3<R, false><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, nullptr,
+                                                                nullptr, T.out_adj, T.out_adj + n, T.direct,
+                                                                lam_out, T.flag);
     } else {
         StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
         adj_prep<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.drho_adj, T.direct, psum);
@@ -1167,6 +1197,34 @@ static rrsph_status launch_steps(rrsph_ctx* ctx, uint32_t nsteps) {
 }
 
 static uint64_t committed_steps(const Ctrl& c) { return c.step + (c.pending ? 1 : 0); }
+
+// One list step with the build decided by the caller (the adjoint's tapes
+// follow their own schedule); `abs_step` is its step index since the upload.
+// A build's sort lands in kbuf / vbuf of the last radix pass.
+static rrsph_status launch_list_step(rrsph_ctx* ctx, bool build, uint64_t abs_step) {
+    if (ctx->use_graph && !ctx->gexec) {
+        rrsph_status st = build_graph(ctx);
+        if (st != RRSPH_OK) return st;
+    }
+    if (ctx->use_graph) HIP_TRY(hipGraphLaunch(build ? ctx->gexec_build : ctx->gexec_list, ctx->stream));
+    else enqueue_list_step(ctx, nullptr, build);
+    HIP_TRY(hipGetLastError());
+    if (build) {
+        const uint32_t fb = (ctx->passes - 1) & 1u;
+        ctx->last_keys = ctx->kbuf[fb];
+        ctx->last_perm = ctx->vbuf[fb];
+        ctx->build_at = abs_step;
+        ++ctx->n_build;
+        ++ctx->n_full;
+    } else {
+        ++ctx->n_list;
+    }
+    ++ctx->launched;
+    ctx->need_full = false;
+    ctx->stepped = true;
+    ctx->fields_valid = true;
+    return RRSPH_OK;
+}
 
 // A step overflowed the incremental re-sort at step index `at`.  Overflows
 // in close succession mean many particles change cell every step: keep to
@@ -1783,36 +1841,78 @@ static rrsph_status restore_state_sync(rrsph_ctx* ctx, const void* src) {
 // k - kslot of seg_base), and the gauges (forward).  Returns the status read
 // after the last step.
 static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base,
-                                uint32_t kslot) {
+                                uint32_t kslot, uint64_t abs0) {
     AdjTape& T = ctx->adj;
+    const size_t n = ctx->P.n;
+    // the forward keeps the states of the last segment: the reverse pass
+    // starts there and needs no replay of it
+    const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
     for (uint32_t k = k0; k < k1; ++k) {
         rrsph_status st;
         if (forward && k % T.every == 0) {
+            if (T.lists && k > 0) {  // the build interval from the displacements so far
+                HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipStreamSynchronize(ctx->stream));
+                if (!ctx->host_ctrl->halted) adapt_list_interval(ctx);
+            }
             st = save_state(ctx, T.ck[k / T.every]);
             if (st != RRSPH_OK) return st;
         }
-        // the forward keeps the states of the last segment: the reverse pass
-        // starts there and needs no replay of it
-        const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
-        if (forward && k >= last0) {
-            st = save_state(ctx, (char*)T.seg + (size_t)(k - last0) * T.blob);
+        const bool keep = forward ? k >= last0 : true;  // seg holds this step's state
+        const size_t slot = forward ? k - last0 : k - kslot;
+        if (keep) {
+            st = save_state(ctx, (char*)seg_base + slot * T.blob);
             if (st != RRSPH_OK) return st;
+            if (!forward && k + 1 == k1) {  // the state before the segment's last step is all the adjoint needs
+                // that step is not launched: a list step reads the lists of the segment's last build; a
+                // step that would build sorts its state in the reverse pass
+                if (T.lists) T.seg_build[slot] = T.builds[k] ? -1 : (int32_t)T.nseg_lists - 1;
+                break;
+            }
         }
-        if (!forward) {
-            st = save_state(ctx, (char*)seg_base + (size_t)(k - kslot) * T.blob);
-            if (st != RRSPH_OK) return st;
-            if (k + 1 == k1) break;  // the state before the segment's last step is all the adjoint needs
+        bool build = false;
+        if (T.lists) {
+            const uint64_t abs = abs0 + (k - k0);
+            build = forward ? (k % T.every == 0 || k == T.force_build || abs - ctx->build_at >= ctx->list_interval)
+                            : T.builds[k] != 0;
+            if (forward) T.builds[k] = build ? 1 : 0;
+            st = launch_list_step(ctx, build, abs);
+        } else {
+            st = launch_steps(ctx, 1);
         }
-        st = launch_steps(ctx, 1);
         if (st != RRSPH_OK) return st;
-        if (!forward || k >= last0) {  // keep the sort of a step whose state seg holds
-            const size_t slot = forward ? k - last0 : k - kslot, n = ctx->P.n;
+        if (keep) {
+            if (slot == 0 || !T.lists) T.nseg_lists = 0;
             CopyPlan plan;
-            plan.add(ctx->last_keys, T.seg_keys + slot * n, n * sizeof(uint32_t));
-            plan.add(ctx->last_perm, T.seg_perm + slot * n, n * sizeof(uint32_t));
+            if (!T.lists || build) {  // keep the sort of a step whose state seg holds
+                plan.add(ctx->last_keys, T.seg_keys + slot * n, n * sizeof(uint32_t));
+                plan.add(ctx->last_perm, T.seg_perm + slot * n, n * sizeof(uint32_t));
+                T.seg_sorted[slot] = 1;
+            } else {
+                T.seg_sorted[slot] = 0;  // its state is in its build's slot order
+            }
+            if (T.lists && build) {  // keep the build's lists and sorted keys
+                if (T.nseg_lists == T.seg_lists.size()) {
+                    AdjTape::Lists L;
+                    hipError_t e = hipSuccess;
+                    L.nbl = dalloc<uint32_t>((size_t)(ctx->P.list_cap + kListPad) * n, e);
+                    L.nbc = dalloc<uint32_t>(n, e);
+                    L.keys = dalloc<uint32_t>(n, e);
+                    T.seg_lists.push_back(L);
+                    if (e != hipSuccess)
+                        return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint lists): ") +
+                                                            hipGetErrorString(e));
+                }
+                const AdjTape::Lists& L = T.seg_lists[T.nseg_lists];
+                HIP_TRY(hipMemcpyAsync(L.nbl, ctx->nbl, (size_t)(ctx->P.list_cap + kListPad) * n * sizeof(uint32_t),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+                plan.add(ctx->nbc, L.nbc, n * sizeof(uint32_t));
+                plan.add(ctx->last_keys, L.keys, n * sizeof(uint32_t));
+                ++T.nseg_lists;
+            }
+            if (T.lists) T.seg_build[slot] = (int32_t)T.nseg_lists - 1;
             st = enqueue_copies(ctx, plan);
             if (st != RRSPH_OK) return st;
-            T.seg_sorted[slot] = 1;
         }
         if (forward) {
             adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec + k);
@@ -1824,26 +1924,40 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
     return sync_status(ctx);
 }
 
-// adjoint_run with the incremental re-sort.  A step that changed more cells
-// than the re-sort takes halts before touching the state, and the launches
-// after it do nothing; clear the halt and resume the range from that step,
-// which runs with the full sort (its record, checkpoint, segment state and
-// gauges are written again).
+// adjoint_run with its halts handled: a step that changed more cells than
+// the incremental re-sort takes, or (forward, list steps) found its lists
+// possibly stale, halts before touching the state, and the launches after
+// it do nothing; clear the halt and resume the range from that step, which
+// runs with the full sort or as a build (its record, checkpoint, segment
+// state and gauges are written again).
 static rrsph_status adjoint_run_checked(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base) {
     HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const uint64_t start = committed_steps(*ctx->host_ctrl);
+    AdjTape& T = ctx->adj;
     uint32_t from = k0;
     while (true) {
-        rrsph_status st = adjoint_run(ctx, from, k1, forward, seg_base, k0);
+        rrsph_status st = adjoint_run(ctx, from, k1, forward, seg_base, k0, start + (from - k0));
         if (st != RRSPH_OK) return st;
         Ctrl& c = *ctx->host_ctrl;
-        if (!(c.halted && c.status == kStatusResort)) return RRSPH_OK;
-        const uint32_t at = k0 + (uint32_t)(committed_steps(c) - start);  // the step that overflowed
-        // the resumed step runs with the full sort, so every resume advances
+        if (!c.halted) return RRSPH_OK;
+        const bool resort = c.status == kStatusResort;
+        const bool stale = c.status == kStatusRebuild && forward && T.lists;
+        if (c.status == kStatusListOverflow)
+            return fail(ctx, RRSPH_ERR_NOT_IMPLEMENTED,
+                        "RRSPH_HIP: a neighbour list ran out of room during the adjoint's tape; run it with "
+                        "RRSPH_HIP_LIST=0");
+        if (!resort && !stale) return RRSPH_OK;
+        const uint32_t at = k0 + (uint32_t)(committed_steps(c) - start);  // the step that halted
+        // the resumed step runs with the full sort or builds, so every resume advances
         if (at < from || at >= k1 || (at == from && from != k0))
-            return fail(ctx, RRSPH_ERR_HIP, "RRSPH_HIP: re-sort overflow outside the replayed range");
-        note_resort_overflow(ctx, committed_steps(c));
+            return fail(ctx, RRSPH_ERR_HIP, "RRSPH_HIP: halt outside the replayed range");
+        if (resort) {
+            note_resort_overflow(ctx, committed_steps(c));
+        } else {
+            note_list_stale(ctx, committed_steps(c));
+            T.force_build = at;
+        }
         c.halted = 0;
         c.status = 0;
         HIP_TRY(hipMemcpyAsync(ctx->ctrl, ctx->host_ctrl, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
@@ -1912,9 +2026,10 @@ rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, 
     return RRSPH_OK;
 }
 
-// The adjoint's tapes and replays run the steps without neighbour lists:
-// a replay must repeat its step exactly, whatever the list schedule was.
-// Entering and leaving that state forces a full sort and a list build.
+// The 2D adjoint's tapes and replays run the steps without neighbour lists
+// (a replay must repeat its step exactly, whatever the list schedule was;
+// the 3D tapes record their schedule instead).  Entering and leaving that
+// state forces a full sort and a list build.
 struct ListSuspend {
     rrsph_ctx* c;
     bool was;
@@ -1936,6 +2051,11 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
 rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                                        const rrsph_gauge_spec* spec, double* eta) {
     if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    if (ctx->P.dim == 3) {  // 3D tapes run list steps when the context does (their schedule recorded)
+        rrsph_status st = adjoint_forward_impl(ctx, nsteps, checkpoint_every, spec, eta);
+        ctx->need_full = true;
+        return st;
+    }
     ListSuspend guard(ctx);
     return adjoint_forward_impl(ctx, nsteps, checkpoint_every, spec, eta);
 }
@@ -1943,6 +2063,11 @@ rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t
 rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
                                         double* grad_r0, double* grad_v0, double* grad_rho0) {
     if (!ctx) return RRSPH_ERR_INVALID_ARGUMENT;
+    if (ctx->P.dim == 3) {
+        rrsph_status st = adjoint_backward_impl(ctx, eta_bar, grad_a, grad_b, grad_r0, grad_v0, grad_rho0);
+        ctx->need_full = true;
+        return st;
+    }
     ListSuspend guard(ctx);
     return adjoint_backward_impl(ctx, eta_bar, grad_a, grad_b, grad_r0, grad_v0, grad_rho0);
 }
@@ -1974,6 +2099,12 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     T.seg_keys = dalloc<uint32_t>(n * T.every, e);
     T.seg_perm = dalloc<uint32_t>(n * T.every, e);
     T.seg_sorted.assign(T.every, 0);
+    T.lists = list_active(ctx) && d3;
+    T.builds.assign(nsteps, 0);
+    T.seg_build.assign(T.every, -1);
+    T.nseg_lists = 0;
+    T.force_build = ~0ull;
+    if (T.lists) T.iota = dalloc<uint32_t>(n, e);
     T.eta = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.zs = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.ws = dalloc<double>((size_t)nsteps * kMaxGauges, e);
@@ -1995,7 +2126,12 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
         free_tape(ctx->adj);
         return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
     }
-    st = adjoint_run_checked(ctx, 0, nsteps, true, nullptr);
+    if (T.lists) {
+        std::vector<uint32_t> id(n);
+        for (size_t i = 0; i < n; ++i) id[i] = (uint32_t)i;
+        HIP_TRY(hipMemcpy(T.iota, id.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    st = adjoint_run_checked(ctx, 0, nsteps, true, T.seg);
     if (st != RRSPH_OK) return st;
     if (ctx->host_ctrl->halted)
         return fail(ctx, RRSPH_ERR_INVALID_ARGUMENT, "RRSPH_HIP: the run halted during the adjoint forward");
@@ -2065,6 +2201,7 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
         // step (the last segment's states are there already after a forward)
         if (!(T.seg_last && seg == (int)nck - 1)) {
             std::fill(T.seg_sorted.begin(), T.seg_sorted.end(), (uint8_t)0);
+            std::fill(T.seg_build.begin(), T.seg_build.end(), -1);
             st = restore_state(ctx, T.ck[seg]);
             if (st != RRSPH_OK) return st;
             st = adjoint_run_checked(ctx, n0, n1, false, T.seg);
@@ -2073,19 +2210,22 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
         T.seg_last = false;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
             const size_t slot = (size_t)(k - n0);
-            const bool sorted = T.seg_sorted[slot] != 0;  // the replay launched step k
-            st = restore_state(ctx, (char*)T.seg + slot * T.blob, !sorted);
+            const bool sorted = T.seg_sorted[slot] != 0;  // the replay launched step k with a sort
+            // list steps: the build whose lists step k read, its state in that build's slot order
+            const int32_t b = T.lists ? T.seg_build[slot] : -1;
+            const AdjTape::Lists* lists = b >= 0 ? &T.seg_lists[(size_t)b] : nullptr;
+            st = restore_state(ctx, (char*)T.seg + slot * T.blob, !sorted && !lists);
             if (st != RRSPH_OK) return st;
             // sort and reorder state k exactly as its step did (predict with its dt)
             adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
-            const uint32_t* sk = sorted ? T.seg_keys + slot * n : nullptr;
-            const uint32_t* sp = sorted ? T.seg_perm + slot * n : nullptr;
+            const uint32_t* sk = sorted ? T.seg_keys + slot * n : (lists ? lists->keys : nullptr);
+            const uint32_t* sp = sorted ? T.seg_perm + slot * n : (lists ? T.iota : nullptr);
             if (ctx->fp32)
                 enqueue_reverse_step<float>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                            T.piston_sum + (size_t)k * kMaxSegments, sk, sp);
+                                            T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists);
             else
                 enqueue_reverse_step<double>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp);
+                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists);
             if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur]);  // loss term of state k
             HIP_TRY(hipGetLastError());
             cur = 1 - cur;
@@ -2123,7 +2263,8 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
         }
         if (grad_rho0) grad_rho0[i] = lam[lw * i + ro];
     }
-    // leave the context as the forward left it
+    // leave the context as the forward left it (restore_state makes the next
+    // step sort, and build when it runs list steps)
     st = restore_state_sync(ctx, T.ck[nck]);
     if (st != RRSPH_OK) return st;
     if (flag) return fail(ctx, RRSPH_ERR_TOO_MANY_NEIGHBOURS, status_message(RRSPH_ERR_TOO_MANY_NEIGHBOURS));
