@@ -5,8 +5,9 @@
 //
 // Adjoint state per particle (original order, 12 doubles):
 //   lam = (r x y z, v x y z, rho, a_prev x y z, drho_prev, 0)
-// Output adjoints of the step (sorted order):
-//   out_a = (a x y z, drho),  out_x = (av x y z, 0)
+// Output adjoints of the step (sorted order), packed by adj_prep3 with the
+// state a neighbour's pair adjoint reads into one record per slot (AdjRec3):
+//   oa = (a x y z, drho),  ox = (av x y z)
 // Gauges: gauge g sits at (x_g, z_g) and weighs fluid particles by
 // w = (1 - ((x - x_g)^2 + (z - z_g)^2) / R^2)^3 inside the disc of radius R;
 // eta_g = y_ref + tau log(sum w e / sum w), e = exp((y - y_ref) / tau).
@@ -20,6 +21,19 @@ namespace rrsph {
 
 constexpr uint32_t kLam3 = 12;
 constexpr uint32_t kDirect3 = 8;
+
+// What the pair adjoint of a neighbour reads of slot s, in one record (one
+// cache line in FP32: the gathers of a hit were six arrays): position and
+// type, the output adjoints, p / R^2, the velocity v and the predicted
+// (u, R) in the step's precision.
+template <class R> struct alignas(32) AdjRec3 {
+    double4 pos;
+    double4 oa;
+    double ox, oy, oz, pw;
+    real4<R> v;  // (v x y z, unused)
+    real4<R> u;  // (u x y z, R)
+};
+static_assert(sizeof(AdjRec3<float>) == 128, "one cache line");
 
 struct GaugeSpec3 {
     GaugeSpec g;
@@ -105,9 +119,8 @@ __global__ void __launch_bounds__(kBlock) state_keys3_kernel(StepParams P, const
 // velocity (one entry per paddle of this step).
 template <class R>
 __global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, StateB3<R> B,
-                                                    const double* __restrict__ lam_in, double4* __restrict__ out_a,
-                                                    double4* __restrict__ out_x, double* __restrict__ direct,
-                                                    double* __restrict__ piston_sum) {
+                                                    const double* __restrict__ lam_in, AdjRec3<R>* __restrict__ rec,
+                                                    double* __restrict__ direct, double* __restrict__ piston_sum) {
     __shared__ double s_p[kMaxSegments][kBlock / kWave];
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     double pist = 0.0;
@@ -150,8 +163,16 @@ __global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, Sta
         }
         if (t == 0) oa.x = oa.y = oa.z = 0.0;  // no neighbours, no acceleration dependence
         oa.w = P.density_treatment != 0 ? rb * vdt + db : 0.0;
-        out_a[s] = oa;
-        out_x[s] = ox;
+        AdjRec3<R> r;
+        r.pos = ps;
+        r.oa = oa;
+        r.ox = ox.x;
+        r.oy = ox.y;
+        r.oz = ox.z;
+        r.pw = (double)B.pw[s];
+        r.v = B.va[s];
+        r.u = B.pred[s];
+        rec[s] = r;
         double* d = direct + kDirect3 * (size_t)s;
         d[0] = rx;
         d[1] = ry;
@@ -253,8 +274,7 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
                                                     const uint32_t* __restrict__ LB,
                                                     const uint32_t* __restrict__ nbl,
                                                     const uint32_t* __restrict__ nbc,
-                                                    const double4* __restrict__ out_a,
-                                                    const double4* __restrict__ out_x,
+                                                    const AdjRec3<R>* __restrict__ rec,
                                                     const double* __restrict__ direct, double* __restrict__ lam_out,
                                                     uint32_t* __restrict__ overflow_flag) {
     __shared__ uint32_t hits[kAdjChunk * kBlock];
@@ -283,8 +303,9 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     const double4 prk = to_d4<R>(B.pred[sc]);  // (u, R)
     const double Pk = B.pw[sc];                // p / R^2
     const double pres_k = B.pres[sc];
-    const double4 oak = out_a[sc];
-    const double4 oxk = out_x[sc];
+    const AdjRec3<R> rk = rec[sc];
+    const double4 oak = rk.oa;
+    const double4 oxk = make_double4(rk.ox, rk.oy, rk.oz, 0.0);
     const int tk = (int)pk.w;
     V3 rb{0, 0, 0}, vb{0, 0, 0}, ub{0, 0, 0};
     double Rb = 0, Pb = 0;
@@ -296,25 +317,17 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
         if (nh == 0) return;
         const uint32_t last = nh - 1;
         uint32_t in = my[0];
-        double4 pn = B.pos[in];
-        real4<R> van = B.va[in], prn = B.pred[in];
-        R pwn = B.pw[in];
-        double4 oan = out_a[in], oxn = out_x[in];
+        AdjRec3<R> rn = rec[in];
 #pragma unroll 1
         for (uint32_t u = 0; u < nh; ++u) {
-            const double4 pi = pn;
-            const double4 vai = to_d4<R>(van);
-            const double4 pri = to_d4<R>(prn);
-            const double Pi = pwn;
-            const double4 oai = oan;
-            const double4 oxi = oxn;
+            const double4 pi = rn.pos;
+            const double4 vai = to_d4<R>(rn.v);
+            const double4 pri = to_d4<R>(rn.u);
+            const double Pi = rn.pw;
+            const double4 oai = rn.oa;
+            const double4 oxi = make_double4(rn.ox, rn.oy, rn.oz, 0.0);
             in = my[min(u + 1, last) * kBlock];
-            pn = B.pos[in];
-            van = B.va[in];
-            prn = B.pred[in];
-            pwn = B.pw[in];
-            oan = out_a[in];
-            oxn = out_x[in];
+            rn = rec[in];
             const int ti = (int)pi.w;
             const V3 e{pi.x - pk.x, pi.y - pk.y, pi.z - pk.z};  // r_i - r_k
             AdjPair Q;

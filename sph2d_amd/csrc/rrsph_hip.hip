@@ -129,7 +129,7 @@ struct AdjTape {
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
     double *eta_bar = nullptr, *partials = nullptr, *piston_sum = nullptr;
     double *lam[2] = {nullptr, nullptr}, *direct = nullptr, *drho_adj = nullptr;
-    double4* out_adj = nullptr;          // 3D: out_a (n) then out_x (n)
+    double4* out_adj = nullptr;          // 2D: out_a (n); 3D: the AdjRec3 records (n)
     uint32_t* flag = nullptr;
     // 3D with neighbour lists: the tape runs list steps.  The forward builds
     // at every checkpoint step and records which steps built; a replay
@@ -960,14 +960,15 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     }
     if (P.dim == 3) {
         StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
-        adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.out_adj + n, T.direct, psum);
+        AdjRec3<R>* rec = (AdjRec3<R>*)T.out_adj;
+        adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, rec, T.direct, psum);
         if (lists)
             adj_pair3<R, true><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, lists->nbl,
-                                                               lists->nbc, T.out_adj, T.out_adj + n, T.direct,
+                                                               lists->nbc, rec, T.direct,
                                                                lam_out, T.flag);
         else
             adj_pair3<R, false><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, nullptr,
-                                                                nullptr, T.out_adj, T.out_adj + n, T.direct,
+                                                                nullptr, rec, T.direct,
                                                                 lam_out, T.flag);
     } else {
         StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
@@ -2115,7 +2116,7 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     T.lam[1] = dalloc<double>((d3 ? kLam3 : 8) * n, e);
     T.direct = dalloc<double>((d3 ? kDirect3 : 5) * n, e);
     T.drho_adj = dalloc<double>(n, e);
-    T.out_adj = dalloc<double4>((d3 ? 2 : 1) * n, e);
+    T.out_adj = dalloc<double4>((d3 ? sizeof(AdjRec3<double>) / sizeof(double4) : 1) * n, e);
     T.flag = dalloc<uint32_t>(1, e);
     if (e != hipSuccess) {
         free_tape(ctx->adj);
