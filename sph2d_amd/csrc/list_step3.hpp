@@ -188,14 +188,29 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
     };
     const int wx0 = wave_min_i32(alive ? x0 : INT_MAX), wx1 = wave_max_i32(alive ? x1 : INT_MIN);
     const int wy0 = wave_min_i32(alive ? y0 : INT_MAX), wy1 = wave_max_i32(alive ? y1 : INT_MIN);
+    const double reach2 = reach * reach;
 #pragma unroll 1
     for (int x = wx0; x <= wx1; ++x) {
+        // distance from the particle to the column's x slab
+        const double xl = P.xmin + x * P.cell;
+        const double dxm = fmax(0.0, fmax(xl - pj.x, pj.x - (xl + P.cell)));
 #pragma unroll 1
         for (int y = wy0; y <= wy1; ++y) {
-            const bool need = x >= x0 && x <= x1 && y >= y0 && y <= y1;
+            const double yl = P.ymin + y * P.cell;
+            const double dym = fmax(0.0, fmax(yl - pj.y, pj.y - (yl + P.cell)));
+            // the z cells of this column within reach: a column off the
+            // particle's own needs a shorter range (corner columns most)
+            const double rem = reach2 - dxm * dxm - dym * dym;
+            bool need = x >= x0 && x <= x1 && y >= y0 && y <= y1 && rem >= 0.0;
+            int za = z0, zb = z1;
+            if (need) {
+                const double zr = sqrt(rem) * (1.0 + 1e-9);
+                za = max(z0, clamped_cell(pj.z - zr, P.zmin, P.cell, P.ncz));
+                zb = min(z1, clamped_cell(pj.z + zr, P.zmin, P.cell, P.ncz));
+            }
             const uint32_t col = need ? ((uint32_t)x * (uint32_t)P.ncy + (uint32_t)y) * (uint32_t)P.ncz : 0u;
-            const uint32_t b = need ? LB[col + z0] : 0u;
-            const uint32_t len = need ? LB[col + z1 + 1] - b : 0u;
+            const uint32_t b = need ? LB[col + za] : 0u;
+            const uint32_t len = need ? LB[col + zb + 1] - b : 0u;
             const uint32_t code_xy = (uint32_t)(x - hx + 2) + 5u * (uint32_t)(y - hy + 2);
             const uint32_t lmax = wave_max_u32(len);
 #pragma unroll 1
