@@ -308,8 +308,8 @@ rrsph_status rrsph_hip_sort_stats(rrsph_ctx* ctx, uint64_t* incremental_steps, u
                                   uint64_t* fallbacks, uint32_t* last_changed);
 
 /* Neighbour-list statistics.  No reference counterpart (the reference
- * builds its lists from scratch every step, GridFind.cpp:112-189): in the 2D
- * cubic configuration the step sorts and walks the cells only on build
+ * builds its lists from scratch every step, GridFind.cpp:112-189): in the
+ * cubic configuration (2D and 3D) the step sorts and walks the cells only on build
  * steps, which list every particle within k h + skin, and the steps between
  * test those lists with the exact cut-off.  Steps that read the lists,
  * build steps, steps that halted because the lists might be stale (re-run
