@@ -321,8 +321,7 @@ def leg_3d(steps, delta, renorm_every=30):
         g.close()
         out[name] = {"value": round(prm.ntotal * steps / wall, 1), "ms_per_step": round(wall / steps * 1e3, 4),
                      "pair_ms": round(per_kernel["pair_kernel"], 4),
-                     "density_sum_ms": round(per_kernel["density_sum_kernel"], 4),
-                     "list_build_ms": round(per_kernel["list_build_kernel"], 4), "list_stats": list_stats,
+                     "density_sum_ms": round(per_kernel["density_sum_kernel"], 4), "list_stats": list_stats,
                      "sort_stats": sort_stats}
     out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
     out["fp32_error_vs_fp64"] = precision_error(states["fp32"], states["fp64"], delta)
