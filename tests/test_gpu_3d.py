@@ -134,3 +134,66 @@ def test_3d_incremental_resort_equals_full_sort(monkeypatch):
     assert out["incremental"][1]["incremental"] == 59
     for k in ("r", "v", "rho", "p", "itype"):
         assert np.array_equal(out["full"][0][k], out["incremental"][0][k]), k
+
+
+@pytest.mark.parametrize("lists", ["1", "0"])
+def test_3d_neighbour_overflow_is_the_reference_error(monkeypatch, lists):
+    """3D, list and sorted steps: more than max_neighbours - 1 neighbours
+    with the consistency check stops with the reference's error
+    (GridFind.cpp:186-188) before whole_step, as the oracle does; the state
+    is the uploaded one."""
+    monkeypatch.setenv("RRSPH_HIP_LIST", lists)
+    lay = G.dam_break_3d(delta=0.04)
+    prm = copy.copy(lay.params)
+    prm.max_neighbours = 16  # a 3D particle has ~75
+    for precision in (H.FP64, H.FP32):
+        gpu = H.HipStep(prm, precision=precision)
+        gpu.upload(lay.r, lay.v, lay.rho, lay.itype)
+        with pytest.raises(H.RrsphError) as e:
+            gpu.step(2)
+        assert e.value.status == 3 and "making neighbours grid: error occured" in str(e.value)
+        t, _dt, step = gpu.time()
+        st = gpu.download()
+        gpu.close()
+        assert step == 0 and t == prm.start_simulation_time
+        assert np.array_equal(st["r"], lay.r) and np.array_equal(st["itype"], lay.itype)
+    ora = oracle.Oracle(prm)
+    ora.set_state(lay.r, lay.v, lay.rho, lay.itype, time=0.0, dt=prm.dt)
+    assert ora.step(1) == 1
+
+
+@pytest.mark.parametrize("lists", ["1", "0"])
+def test_3d_fix_turns_escaping_fluid_into_non_existent_particles(monkeypatch, lists):
+    """3D FIX (consistency_treatment = 2), list and sorted steps: four
+    isolated fluid particles above the column fly out of the domain and
+    become itype 0; the rest of the state follows the oracle."""
+    from sph2d_amd import params as P
+
+    monkeypatch.setenv("RRSPH_HIP_LIST", lists)
+    lay = G.dam_break_3d()
+    prm = copy.copy(lay.params)
+    prm.consistency_treatment = P.CONSISTENCY_FIX
+    nf = prm.nfluid
+    extra_r = np.array([[0.2, 0.9, 0.2], [0.5, 0.95, 0.1], [0.7, 0.9, 0.3], [0.3, 0.92, 0.35]])
+    extra_v = np.array([[0.0, 900.0, 0.0], [0.0, 1200.0, 0.0], [3000.0, 0.0, 0.0], [0.0, 500.0, -3000.0]])
+    r = np.concatenate([lay.r[:nf], extra_r, lay.r[nf:]])
+    v = np.concatenate([lay.v[:nf], extra_v, lay.v[nf:]])
+    rho = np.concatenate([lay.rho[:nf], np.full(4, prm.rho0), lay.rho[nf:]])
+    itype = np.concatenate([lay.itype[:nf], np.full(4, P.TYPE_WATER, dtype=np.int32), lay.itype[nf:]])
+    prm.nfluid, prm.ntotal = nf + 4, prm.ntotal + 4
+    prm.nwm_particles_start = prm.nwm_particles_end = prm.ntotal
+    gpu = H.HipStep(prm, precision=H.FP64)
+    gpu.upload(r, v, rho, itype)
+    gpu.step(6)
+    got = gpu.download()
+    gpu.close()
+    ora = oracle.Oracle(prm)
+    ora.set_state(r, v, rho, itype, time=0.0, dt=prm.dt)
+    assert ora.step(6) == 0
+    ref = ora.state()
+    assert np.all(got["itype"][nf:nf + 4] == 0), got["itype"][nf:nf + 4]
+    assert np.array_equal(got["itype"], ref["itype"])
+    assert np.array_equal(got["r"][nf:nf + 4], ref["r"][nf:nf + 4])  # dead particles stay where they died
+    alive = ref["itype"] != 0
+    assert np.max(np.abs(got["r"][alive] - ref["r"][alive])) < 1e-12
+    assert np.max(np.abs(got["rho"][alive] - ref["rho"][alive]) / ref["rho"][alive]) < 1e-11
