@@ -215,13 +215,19 @@ __device__ __forceinline__ double fourier_term(const StepParams& P, double t, in
     return kw * (P.piston_a[seg * P.piston_nharm + k - 1] * cs - P.piston_b[seg * P.piston_nharm + k - 1] * sn);
 }
 
-// The step prologue, run by every thread of one block on an LDS copy of the
-// control block: the block loads it and writes it back in one round of
-// coalesced accesses (a chain of dependent global accesses from one thread
-// costs about half a microsecond each), and the Fourier piston's terms are
-// evaluated one per thread, then summed by thread 0 in harmonic order.
-// `before` and `after` run on thread 0 around the commit, on the copy.
-constexpr int kPistonTerms = 32;  // paddles x harmonics of the Fourier piston
+// The step prologue, run by the first wave of the block with the control
+// block in scalar registers: each of the first kCtrlWords lanes loads one
+// 8-byte word (with the kernel's other loads), every lane reads them all
+// with readlane (the values are uniform), the wave commits the previous
+// step and makes this step's snapshot on that copy, the Fourier piston's
+// terms are evaluated one per lane and summed in harmonic order, and lane
+// i writes word i back.  (Updating the block from one thread through
+// memory, LDS or global, is a chain of dependent round trips.)  `before`
+// and `after` act on the copy.  Every thread of the block calls it; the
+// barrier at the end orders the write-back before the block's later
+// accesses to the control block.
+constexpr int kPistonTerms = 32;  // paddles x harmonics of the Fourier piston (at most one per lane)
+static_assert(kPistonTerms <= kWave, "the piston's terms are evaluated one per lane");
 constexpr uint32_t kCtrlWords = sizeof(Ctrl) / 8;
 static_assert(sizeof(Ctrl) % 8 == 0 && kCtrlWords <= kWave, "the control block is copied in 8-byte words");
 
@@ -235,43 +241,47 @@ __device__ __forceinline__ unsigned long long ctrl_word(const Ctrl* ctrl) {
 template <class B, class A>
 __device__ __forceinline__ void block_prologue(const StepParams& P, Ctrl* ctrl, unsigned long long word, B&& before,
                                                A&& after) {
-    __shared__ unsigned long long s_cw[kCtrlWords];
-    __shared__ double s_term[kPistonTerms];
-    __shared__ uint32_t s_go;
-    Ctrl& c = *reinterpret_cast<Ctrl*>(s_cw);
-    if (threadIdx.x < kCtrlWords) s_cw[threadIdx.x] = word;
-    __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kWave) {
+        union {
+            Ctrl c;
+            unsigned long long w[kCtrlWords];
+        } u;
+        const int lo = (int)(uint32_t)word, hi = (int)(uint32_t)(word >> 32);
+#pragma unroll
+        for (uint32_t i = 0; i < kCtrlWords; ++i)
+            u.w[i] = (unsigned long long)(uint32_t)__builtin_amdgcn_readlane(lo, i) |
+                     ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane(hi, i) << 32);
+        Ctrl& c = u.c;
         before(c);
-        s_go = prologue_commit(P, c) ? 1u : 0u;
-    }
-    __syncthreads();
-    const bool fourier = s_go && c.piston_active && P.nwm == 6;
-    const int nterms = max(P.piston_segments, 1) * P.piston_nharm;
-    if (fourier)
-        for (int t = threadIdx.x; t < nterms; t += blockDim.x)
-            s_term[t] = fourier_term(P, c.time, t / P.piston_nharm, t % P.piston_nharm + 1);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_go) {
+        if (prologue_commit(P, c)) {
             double vseg[kMaxSegments];
+#pragma unroll
             for (int g = 0; g < kMaxSegments; ++g) vseg[g] = 0.0;
-            if (fourier) {
-                for (int g = 0; g < max(P.piston_segments, 1); ++g) {
+            if (c.piston_active && P.nwm == 6) {
+                const int nh = P.piston_nharm, nseg = max(P.piston_segments, 1);
+                const int lane = (int)threadIdx.x;
+                const double term = lane < nseg * nh ? fourier_term(P, c.time, lane / nh, lane % nh + 1) : 0.0;
+#pragma unroll
+                for (int g = 0; g < kMaxSegments; ++g) {
                     double v = 0;
-                    for (int k = 0; k < P.piston_nharm; ++k) v += s_term[g * P.piston_nharm + k];
+                    if (g < nseg)
+                        for (int k = 0; k < nh; ++k) v += __shfl(term, g * nh + k);
                     vseg[g] = v;
                 }
             } else if (c.piston_active) {
                 vseg[0] = piston_velocity(P, c.time);
             }
             c.piston_v = vseg[0];
+#pragma unroll
             for (int g = 0; g < kMaxSegments; ++g) c.piston_vseg[g] = g < P.piston_segments ? vseg[g] : 0.0;
         }
         after(c);
+        unsigned long long out = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kCtrlWords; ++i) out = threadIdx.x == i ? u.w[i] : out;
+        if (threadIdx.x < kCtrlWords) reinterpret_cast<unsigned long long*>(ctrl)[threadIdx.x] = out;
     }
     __syncthreads();
-    if (threadIdx.x < kCtrlWords) reinterpret_cast<unsigned long long*>(ctrl)[threadIdx.x] = s_cw[threadIdx.x];
 }
 
 // The prologue of a sorted step (one block of kWave threads).
