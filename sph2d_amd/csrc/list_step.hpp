@@ -287,7 +287,7 @@ __device__ __forceinline__ bool list_cut(const StepParams& P, const CutFP32& C, 
     dx = (ni.x - nj.x) + o.x;
     dy = (ni.y - nj.y) + o.y;
     const float d2 = dx * dx + dy * dy;
-    if (!(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
+    if (!(d2 > C.lo && d2 < C.hi)) return d2 <= C.lo;  // on the band's edge: within a rounding of lo < cut2
     const double2 q = pos[e & kSlotMask];  // rare: the band around the cut-off
     const double ddx = __dsub_rn(q.x, pj.x), ddy = __dsub_rn(q.y, pj.y);
     dx = (float)ddx;

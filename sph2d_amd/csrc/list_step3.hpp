@@ -306,7 +306,7 @@ __device__ __forceinline__ bool list_cut3(const StepParams& P, const CutFP32& C,
     dy = (ni.y - nj.y) + o.y;
     dz = (ni.z - nj.z) + o.z;
     const float d2 = dx * dx + dy * dy + dz * dz;
-    if (!(d2 > C.lo && d2 < C.hi)) return d2 < C.lo;
+    if (!(d2 > C.lo && d2 < C.hi)) return d2 <= C.lo;  // on the band's edge: within a rounding of lo < cut2
     const double4 q = pos[e & kSlotMask3];  // rare: the band around the cut-off
     const double ddx = __dsub_rn(q.x, pj.x), ddy = __dsub_rn(q.y, pj.y), ddz = __dsub_rn(q.z, pj.z);
     dx = (float)ddx;

@@ -34,7 +34,7 @@ def _fma32(a, b, c):
     return _f32(np.asarray(a, np.float64) * np.asarray(b, np.float64) + np.asarray(c, np.float64))
 
 
-def _entries(seed, x0=9.9, x1=10.1, near_cut=0):
+def _entries(seed, x0=9.9, x1=10.1, near_cut=0, edge=None):
     lay = G.piston_tank(delta=G.c1_1m_delta())
     prm = lay.params
     cell = prm.cell_scale_k * prm.hsml
@@ -52,12 +52,16 @@ def _entries(seed, x0=9.9, x1=10.1, near_cut=0):
     mag = rng.uniform(0, 1, len(rb)) * 0.5 * SKIN * cell * (1 - 1e-9)
     r = rb + np.stack([mag * np.cos(ang), mag * np.sin(ang)], axis=1)
     if near_cut:
-        # move the second particle of some pairs to within a few ulps of the cut-off
+        # move the second particle of some pairs to within a few ulps of the
+        # cut-off, or of an edge of the band (its FP32 value) with edge="lo"/"hi"
+        dist = cell
+        if edge:
+            dist = np.sqrt(float(np.float32(cell * cell * (1 - BAND if edge == "lo" else 1 + BAND))))
         idx = rng.choice(len(pairs), size=near_cut, replace=False)
         for i, j in pairs[idx]:
             d = r[j] - r[i]
             d /= np.linalg.norm(d)
-            r[j] = r[i] + d * cell * (1 + rng.uniform(-1e-12, 1e-12))
+            r[j] = r[i] + d * dist * (1 + rng.uniform(-1e-12, 1e-12))
     return r, hx, hy, pairs, cell, xmin, ymin
 
 
@@ -96,7 +100,7 @@ def test_fp32_entry_decisions_outside_the_band_are_the_fp64_tests(fused):
     assert rel.max() < BAND / 10  # a tenth of the band at most
     lo, hi = np.float32(cut2 * (1 - BAND)), np.float32(cut2 * (1 + BAND))
     outside = ~((d2 > lo) & (d2 < hi))
-    assert np.array_equal((d2 < lo)[outside], (d2e < cut2)[outside])
+    assert np.array_equal((d2 <= lo)[outside], (d2e < cut2)[outside])
 
 
 def test_pairs_at_the_cut_off_fall_in_the_band():
@@ -108,3 +112,23 @@ def test_pairs_at_the_cut_off_fall_in_the_band():
         assert at_cut.sum() >= 1000
         lo, hi = np.float32(cut2 * (1 - BAND)), np.float32(cut2 * (1 + BAND))
         assert np.all((d2[at_cut] > lo) & (d2[at_cut] < hi))
+
+
+@pytest.mark.parametrize("edge", ["lo", "hi"])
+def test_pairs_on_the_band_edges_are_decided_as_the_fp64_test(edge):
+    """Pairs placed at the FP32 value of a band edge: some FP32 squared
+    distances round exactly onto the edge, which lies outside the open band
+    (lo, hi), so the FP32 value decides them.  d2 == lo must count as within
+    the cut-off and d2 == hi as outside (the exact value lies within a
+    rounding of the edge, a 1e-5 of the cut-off away from it): the decision
+    is d2 <= lo, as the list steps take it."""
+    r, hx, hy, pairs, cell, xmin, ymin = _entries(seed=13, near_cut=4000, edge=edge)
+    for fused in (False, True):
+        d2, d2e, far, cut2 = _decide(r, hx, hy, pairs, cell, xmin, ymin, fused)
+        lo, hi = np.float32(cut2 * (1 - BAND)), np.float32(cut2 * (1 + BAND))
+        on = d2 == np.float64(lo if edge == "lo" else hi)
+        print(edge, "fused" if fused else "separate", "pairs on the edge", int(on.sum()))
+        assert on.sum() >= 20
+        assert np.all((d2e < cut2)[on] == (edge == "lo"))
+        outside = ~((d2 > lo) & (d2 < hi))
+        assert np.array_equal((d2 <= lo)[outside], (d2e < cut2)[outside])
