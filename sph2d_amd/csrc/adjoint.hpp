@@ -399,8 +399,10 @@ __global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, Stat
                                                    const double* __restrict__ direct, double* __restrict__ lam_out,
                                                    uint32_t* __restrict__ overflow_flag) {
     __shared__ uint32_t hits[kAdjChunk * kBlock];
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= P.n) return;
+    // XCD-aware: each XCD walks one contiguous slice of the slots
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    const uint32_t s = lb * kBlock + threadIdx.x;
+    if (lb >= P.nblocks || s >= P.n) return;
     AdjConsts K;
     K.h = P.h;
     K.inv_h = 1.0 / P.h;
@@ -465,28 +467,32 @@ __global__ void __launch_bounds__(kBlock) adj_pair(StepParams P, AdjStep T, Stat
                 const double coef = pri.w + prk.w + Q.A;
                 const double dvx = vai.x - vak.x, dvy = vai.y - vak.y;  // v_i - v_k
                 double ebx, eby, dvbx, dvby, dubx, duby, irb, coefb;
-                // k owns the sum (e = r_i - r_k, dv = v_i - v_k, du = u_i - u_k)
-                adj_orient(K, Q, ex, ey, dvx, dvy, dux, duy, K.m * oak.x, K.m * oak.y, K.m * dak, K.xf * oak.z,
-                           K.xf * oak.w, oak.x, oak.y, K.lj_on && tk > 0 && ti < 0 && Q.d < K.r0,
-                           K.xsph_on && ti > 0, coef, ebx, eby, dvbx, dvby, dubx, duby, irb, coefb);
+                // Both orientations of the pair in one evaluation.  k's sum at
+                // k's adjoints is adj_orient at (e, dv, du); i's sum at i's
+                // adjoints is adj_orient at (-e, -dv, -du), entering rb, vb, ub
+                // with the opposite sign and Rb, Pb with the same.  Every
+                // output is linear in the owner's adjoints, and even (the a-
+                // and av-adjoint terms, the wall force) or odd (the
+                // drho-adjoint terms) in (e, dv, du), so the two make one
+                // evaluation at (e, dv, du) with the owner adjoints
+                // m (oa_k - oa_i) and m (drho_k + drho_i), the XSPH adjoint of
+                // whichever owner has a fluid neighbour (i's negated), and the
+                // wall force of whichever owner is the fluid particle (i's
+                // negated).
+                const bool xk = K.xsph_on && ti > 0, xi = K.xsph_on && tk > 0;
+                const bool ljk = K.lj_on && tk > 0 && ti < 0 && Q.d < K.r0;
+                const bool lji = K.lj_on && ti > 0 && tk < 0 && Q.d < K.r0;
+                const double xdx = K.xf * ((xk ? oak.z : 0.0) - (xi ? oai.z : 0.0));
+                const double xdy = K.xf * ((xk ? oak.w : 0.0) - (xi ? oai.w : 0.0));
+                adj_orient(K, Q, ex, ey, dvx, dvy, dux, duy, K.m * (oak.x - oai.x), K.m * (oak.y - oai.y),
+                           K.m * (dak + dai), xdx, xdy, ljk ? oak.x : -oai.x, ljk ? oak.y : -oai.y, ljk || lji,
+                           xk || xi, coef, ebx, eby, dvbx, dvby, dubx, duby, irb, coefb);
                 rbx -= ebx;
                 rby -= eby;
                 vbx -= dvbx;
                 vby -= dvby;
                 ubx -= dubx;
                 uby -= duby;
-                Rb -= Q.IR * Q.IR * irb;
-                Pb += coefb;
-                // i owns the sum (e = r_k - r_i, dv = v_k - v_i, du = u_k - u_i)
-                adj_orient(K, Q, -ex, -ey, -dvx, -dvy, -dux, -duy, K.m * oai.x, K.m * oai.y, K.m * dai,
-                           K.xf * oai.z, K.xf * oai.w, oai.x, oai.y, K.lj_on && ti > 0 && tk < 0 && Q.d < K.r0,
-                           K.xsph_on && tk > 0, coef, ebx, eby, dvbx, dvby, dubx, duby, irb, coefb);
-                rbx += ebx;
-                rby += eby;
-                vbx += dvbx;
-                vby += dvby;
-                ubx += dubx;
-                uby += duby;
                 Rb -= Q.IR * Q.IR * irb;
                 Pb += coefb;
             }

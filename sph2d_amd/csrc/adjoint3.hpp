@@ -278,7 +278,10 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
                                                     const double* __restrict__ direct, double* __restrict__ lam_out,
                                                     uint32_t* __restrict__ overflow_flag) {
     __shared__ uint32_t hits[kAdjChunk * kBlock];
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    // XCD-aware: each XCD walks one contiguous slice of the slots
+    const uint32_t lb = xcd_block(P.blocks_per_xcd);
+    if (lb >= P.nblocks) return;
+    const uint32_t s = lb * kBlock + threadIdx.x;
     const bool live = s < P.n;
     const uint32_t sc = live ? s : P.n - 1;
     AdjConsts K;
@@ -312,7 +315,8 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     bool overflow = false;
     uint32_t* my = hits + threadIdx.x;
     // the loads of hit u + 1 are issued before the arithmetic of hit u (the
-    // kernel runs at 2 waves per SIMD either way, so the registers are free)
+    // kernel runs at 2 waves per SIMD either way, so the registers are free;
+    // a second record in flight needs more registers than 2 waves leave)
     auto flush = [&](uint32_t nh) {
         if (nh == 0) return;
         const uint32_t last = nh - 1;
@@ -356,11 +360,26 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
             const V3 dv{vai.x - vak.x, vai.y - vak.y, vai.z - vak.z};  // v_i - v_k
             V3 eb, dvb, dub;
             double irb, coefb;
-            // k owns the sum (e = r_i - r_k, dv = v_i - v_k, du = u_i - u_k)
-            adj_orient3(K, Q, e, dv, du, V3{K.m * oak.x, K.m * oak.y, K.m * oak.z}, K.m * oak.w,
-                        V3{K.xf * oxk.x, K.xf * oxk.y, K.xf * oxk.z}, V3{oak.x, oak.y, oak.z},
-                        K.lj_on && tk > 0 && ti < 0 && Q.d < K.r0, K.xsph_on && ti > 0, coef, eb, dvb, dub, irb,
-                        coefb);
+            // Both orientations of the pair in one evaluation.  k's sum at
+            // k's adjoints is adj_orient3 at (e, dv, du); i's sum at i's
+            // adjoints is adj_orient3 at (-e, -dv, -du), entering rb, vb, ub
+            // with the opposite sign and Rb, Pb with the same.  Every output is
+            // linear in the owner's adjoints, and even (the a- and
+            // av-adjoint terms, the wall force) or odd (the drho-adjoint
+            // terms) in (e, dv, du), so the two make one evaluation at
+            // (e, dv, du) with the owner adjoints m (oa_k - oa_i) and
+            // m (drho_k + drho_i), the XSPH adjoint of whichever owner has a
+            // fluid neighbour (i's negated), and the wall force of whichever
+            // owner is the fluid particle (i's negated).
+            const bool xk = K.xsph_on && ti > 0, xi = K.xsph_on && tk > 0;
+            const bool ljk = K.lj_on && tk > 0 && ti < 0 && Q.d < K.r0;
+            const bool lji = K.lj_on && ti > 0 && tk < 0 && Q.d < K.r0;
+            const V3 Xd{K.xf * ((xk ? oxk.x : 0.0) - (xi ? oxi.x : 0.0)),
+                        K.xf * ((xk ? oxk.y : 0.0) - (xi ? oxi.y : 0.0)),
+                        K.xf * ((xk ? oxk.z : 0.0) - (xi ? oxi.z : 0.0))};
+            const V3 L = ljk ? V3{oak.x, oak.y, oak.z} : V3{-oai.x, -oai.y, -oai.z};
+            adj_orient3(K, Q, e, dv, du, V3{K.m * (oak.x - oai.x), K.m * (oak.y - oai.y), K.m * (oak.z - oai.z)},
+                        K.m * (oak.w + oai.w), Xd, L, ljk || lji, xk || xi, coef, eb, dvb, dub, irb, coefb);
             rb.x -= eb.x;
             rb.y -= eb.y;
             rb.z -= eb.z;
@@ -370,23 +389,6 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
             ub.x -= dub.x;
             ub.y -= dub.y;
             ub.z -= dub.z;
-            Rb -= Q.IR * Q.IR * irb;
-            Pb += coefb;
-            // i owns the sum (e = r_k - r_i, dv = v_k - v_i, du = u_k - u_i)
-            adj_orient3(K, Q, V3{-e.x, -e.y, -e.z}, V3{-dv.x, -dv.y, -dv.z}, V3{-du.x, -du.y, -du.z},
-                        V3{K.m * oai.x, K.m * oai.y, K.m * oai.z}, K.m * oai.w,
-                        V3{K.xf * oxi.x, K.xf * oxi.y, K.xf * oxi.z}, V3{oai.x, oai.y, oai.z},
-                        K.lj_on && ti > 0 && tk < 0 && Q.d < K.r0, K.xsph_on && tk > 0, coef, eb, dvb, dub, irb,
-                        coefb);
-            rb.x += eb.x;
-            rb.y += eb.y;
-            rb.z += eb.z;
-            vb.x += dvb.x;
-            vb.y += dvb.y;
-            vb.z += dvb.z;
-            ub.x += dub.x;
-            ub.y += dub.y;
-            ub.z += dub.z;
             Rb -= Q.IR * Q.IR * irb;
             Pb += coefb;
         }

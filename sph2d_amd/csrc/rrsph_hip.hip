@@ -951,6 +951,7 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     const StepParams& P = ctx->P;
     const size_t n = P.n;
     const uint32_t nb = grid_blocks(P.n, kBlock);
+    const uint32_t nb_xcd = P.blocks_per_xcd * kNumXcd;  // the pair gathers map blocks to XCDs
     if (keys) {  // the sort step k ran with, saved by the replay: reorder only
         ctx->sorted_keys = keys;
         ctx->sorted_perm = perm;
@@ -963,17 +964,15 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
         AdjRec3<R>* rec = (AdjRec3<R>*)T.out_adj;
         adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, rec, T.direct, psum);
         if (lists)
-            adj_pair3<R, true><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, lists->nbl,
-                                                               lists->nbc, rec, T.direct,
-                                                               lam_out, T.flag);
+            adj_pair3<R, true><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, lists->nbl,
+                                                                   lists->nbc, rec, T.direct, lam_out, T.flag);
         else
-            adj_pair3<R, false><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, nullptr,
-                                                                nullptr, rec, T.direct,
-                                                                lam_out, T.flag);
+            adj_pair3<R, false><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, nullptr,
+                                                                    nullptr, rec, T.direct, lam_out, T.flag);
     } else {
         StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
         adj_prep<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.drho_adj, T.direct, psum);
-        adj_pair<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
+        adj_pair<R><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, T.out_adj, T.drho_adj,
                                                     T.direct, lam_out, T.flag);
     }
 }
