@@ -22,18 +22,48 @@ namespace rrsph {
 constexpr uint32_t kLam3 = 12;
 constexpr uint32_t kDirect3 = 8;
 
-// What the pair adjoint of a neighbour reads of slot s, in one record (one
-// cache line in FP32: the gathers of a hit were six arrays): position and
-// type, the output adjoints, p / R^2, the velocity v and the predicted
-// (u, R) in the step's precision.
-template <class R> struct alignas(32) AdjRec3 {
+// What the pair adjoint of a neighbour reads of slot s, in one record:
+// the output adjoints oa = (a x y z, drho) and ox = (av x y z), p / R^2,
+// the velocity v and the predicted (u, R).  The FP64 step's record holds
+// its position too, all in FP64.  The FP32 step's record is 64 bytes, two
+// slots per cache line (a hit's gather costs by the lines it touches, and
+// the neighbours of a particle lie in runs of slots): the state in its
+// FP32 and the adjoints rounded to FP32 (the arithmetic stays FP64); the
+// FP64 position comes from the packed position array, whose lines the
+// neighbour walk has just read.
+template <class R> struct AdjRec3;
+template <> struct alignas(32) AdjRec3<double> {
     double4 pos;
     double4 oa;
     double ox, oy, oz, pw;
-    real4<R> v;  // (v x y z, unused)
-    real4<R> u;  // (u x y z, R)
+    double4 v;  // (v x y z, unused)
+    double4 u;  // (u x y z, R)
 };
-static_assert(sizeof(AdjRec3<float>) == 128, "one cache line");
+template <> struct alignas(64) AdjRec3<float> {
+    float4 vp;  // (v x y z, p / R^2)
+    float4 u;   // (u x y z, R)
+    float4 oa;
+    float4 ox;  // (av x y z, unused)
+};
+static_assert(sizeof(AdjRec3<float>) == 64, "two records per cache line");
+
+// The fields of a record, widened to FP64 (pos: the FP64 record's own,
+// else the position array's entry).
+struct AdjHit3 {
+    double4 pos, v, u, oa, ox;
+    double pw;
+};
+__device__ __forceinline__ AdjHit3 adj_hit3(const AdjRec3<double>& r, const double4&) {
+    return AdjHit3{r.pos, r.v, r.u, r.oa, make_double4(r.ox, r.oy, r.oz, 0.0), r.pw};
+}
+__device__ __forceinline__ AdjHit3 adj_hit3(const AdjRec3<float>& r, const double4& pos) {
+    return AdjHit3{pos,
+                   make_double4(r.vp.x, r.vp.y, r.vp.z, 0.0),
+                   make_double4(r.u.x, r.u.y, r.u.z, r.u.w),
+                   make_double4(r.oa.x, r.oa.y, r.oa.z, r.oa.w),
+                   make_double4(r.ox.x, r.ox.y, r.ox.z, 0.0),
+                   (double)r.vp.w};
+}
 
 struct GaugeSpec3 {
     GaugeSpec g;
@@ -164,14 +194,22 @@ __global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, Sta
         if (t == 0) oa.x = oa.y = oa.z = 0.0;  // no neighbours, no acceleration dependence
         oa.w = P.density_treatment != 0 ? rb * vdt + db : 0.0;
         AdjRec3<R> r;
-        r.pos = ps;
-        r.oa = oa;
-        r.ox = ox.x;
-        r.oy = ox.y;
-        r.oz = ox.z;
-        r.pw = (double)B.pw[s];
-        r.v = B.va[s];
-        r.u = B.pred[s];
+        if constexpr (sizeof(R) == 4) {
+            const real4<R> va = B.va[s];
+            r.vp = make_float4(va.x, va.y, va.z, B.pw[s]);
+            r.u = B.pred[s];
+            r.oa = make_float4((float)oa.x, (float)oa.y, (float)oa.z, (float)oa.w);
+            r.ox = make_float4((float)ox.x, (float)ox.y, (float)ox.z, 0.f);
+        } else {
+            r.pos = ps;
+            r.oa = oa;
+            r.ox = ox.x;
+            r.oy = ox.y;
+            r.oz = ox.z;
+            r.pw = B.pw[s];
+            r.v = B.va[s];
+            r.u = B.pred[s];
+        }
         rec[s] = r;
         double* d = direct + kDirect3 * (size_t)s;
         d[0] = rx;
@@ -306,9 +344,9 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     const double4 prk = to_d4<R>(B.pred[sc]);  // (u, R)
     const double Pk = B.pw[sc];                // p / R^2
     const double pres_k = B.pres[sc];
-    const AdjRec3<R> rk = rec[sc];
-    const double4 oak = rk.oa;
-    const double4 oxk = make_double4(rk.ox, rk.oy, rk.oz, 0.0);
+    const AdjHit3 hk = adj_hit3(rec[sc], pk);
+    const double4 oak = hk.oa;
+    const double4 oxk = hk.ox;
     const int tk = (int)pk.w;
     V3 rb{0, 0, 0}, vb{0, 0, 0}, ub{0, 0, 0};
     double Rb = 0, Pb = 0;
@@ -322,16 +360,20 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
         const uint32_t last = nh - 1;
         uint32_t in = my[0];
         AdjRec3<R> rn = rec[in];
+        double4 pn{};
+        if constexpr (sizeof(R) == 4) pn = B.pos[in];
 #pragma unroll 1
         for (uint32_t u = 0; u < nh; ++u) {
-            const double4 pi = rn.pos;
-            const double4 vai = to_d4<R>(rn.v);
-            const double4 pri = to_d4<R>(rn.u);
-            const double Pi = rn.pw;
-            const double4 oai = rn.oa;
-            const double4 oxi = make_double4(rn.ox, rn.oy, rn.oz, 0.0);
+            const AdjHit3 h = adj_hit3(rn, pn);
+            const double4 pi = h.pos;
+            const double4 vai = h.v;
+            const double4 pri = h.u;
+            const double Pi = h.pw;
+            const double4 oai = h.oa;
+            const double4 oxi = h.ox;
             in = my[min(u + 1, last) * kBlock];
             rn = rec[in];
+            if constexpr (sizeof(R) == 4) pn = B.pos[in];
             const int ti = (int)pi.w;
             const V3 e{pi.x - pk.x, pi.y - pk.y, pi.z - pk.z};  // r_i - r_k
             AdjPair Q;
