@@ -238,10 +238,11 @@ __global__ void __launch_bounds__(kBlock) list_build_kernel(StepParams P, Ctrl* 
                     const double dx = q[u].x - pj.x, dy = q[u].y - pj.y;
                     const bool in = (i + u < e) & (i + u != s) & (dx * dx + dy * dy < P.list_r2);
                     if (in) {
-                        // the offset of the neighbour's home cell (the rows of a
-                        // column range are consecutive keys)
-                        const uint32_t ki = skey[i + u];
-                        const int dcx = col - hx, dcy = (int)ki - col * P.ncy - hy;
+                        // the offset of the neighbour's home cell: column col, and the
+                        // row of its key, from the same position by the same rule
+                        // (cell_key), without loading the key
+                        uint32_t ybad = 0;
+                        const int dcx = col - hx, dcy = cell_coord(q[u].y, P.ymin, P.cell, P.ncy, ybad) - hy;
                         bad |= dcx < -3 || dcx > 4 || dcy < -3 || dcy > 4;
                         const uint32_t code = (uint32_t)(dcx + 3) | ((uint32_t)(dcy + 3) << 3);
                         if (cnt < P.list_cap) nbl[(size_t)cnt * P.n + s] = (i + u) | (code << kSlotBits);

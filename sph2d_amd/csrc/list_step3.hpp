@@ -223,8 +223,10 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
                     const double dx = q[u].x - pj.x, dy = q[u].y - pj.y, dz = q[u].z - pj.z;
                     const bool in = (i + u < len) & (b + i + u != s) & (dx * dx + dy * dy + dz * dz < P.list_r2);
                     if (in) {
-                        // the z cell of the neighbour (a column's z cells are consecutive keys)
-                        const int dcz = (int)(skey[b + i + u] - col) - hz;
+                        // the z cell of the neighbour: its key's, from the same position by the
+                        // same rule (cell_key3), without loading the key
+                        uint32_t zbad = 0;
+                        const int dcz = cell_coord(q[u].z, P.zmin, P.cell, P.ncz, zbad) - hz;
                         const uint32_t code = code_xy + 25u * (uint32_t)(dcz + 2);
                         my[nh * kBlock] = (b + i + u) | (code << kSlotBits3);
                         ++nh;
