@@ -193,10 +193,12 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_grad(uint32_t n, const do
     const double2 p = pos[s];
     const double inv_r2 = 1.0 / (G.radius * G.radius);
     double gx = 0, gy = 0;
+    bool seen = false;  // inside a window: only those particles update lam
     for (uint32_t g = 0; g < G.n; ++g) {
         const double dx = p.x - G.x[g];
         const double u = 1.0 - dx * dx * inv_r2;
         if (!(u > 0.0) || !(wsum[g] > 0.0) || eta_bar[g] == 0.0) continue;
+        seen = true;
         const double w = u * u * u;
         const double wp = -6.0 * dx * inv_r2 * u * u;  // dw/dx
         const double e = exp((p.y - G.y_ref) / G.tau);
@@ -204,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_grad(uint32_t n, const do
         gy += eta_bar[g] * w * e / z;
         gx += eta_bar[g] * G.tau * wp * (e / z - 1.0 / ws);
     }
+    if (!seen) return;
     double* l = lam + 8 * (size_t)oid[s];
     l[0] += gx;
     l[1] += gy;

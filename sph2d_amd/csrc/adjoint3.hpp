@@ -115,10 +115,12 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_grad3(uint32_t n, const d
     if ((int)p.w != G.g.ptype) return;
     const double inv_r2 = 1.0 / (G.g.radius * G.g.radius);
     double gx = 0, gy = 0, gz = 0;
+    bool seen = false;  // inside a window: only those particles update lam
     for (uint32_t g = 0; g < G.g.n; ++g) {
         double dx, dz;
         const double u = gauge_weight3(G, g, p.x, p.z, inv_r2, dx, dz);
         if (!(u > 0.0) || !(wsum[g] > 0.0) || eta_bar[g] == 0.0) continue;
+        seen = true;
         const double w = u * u * u;
         const double wr = -6.0 * inv_r2 * u * u;  // dw/dx = wr dx, dw/dz = wr dz
         const double e = exp((p.y - G.g.y_ref) / G.g.tau);
@@ -128,6 +130,7 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_grad3(uint32_t n, const d
         gx += c * dx;
         gz += c * dz;
     }
+    if (!seen) return;
     double* l = lam + kLam3 * (size_t)oid[s];
     l[0] += gx;
     l[1] += gy;
