@@ -579,25 +579,34 @@ rrsph_status build_graph(rrsph_ctx* ctx) {
 
 // Sort + reorder of the current state into B (with a scratch pred) for the
 // utility queries, using a control block that is never halted.
-template <class R> void enqueue_util_reorder(rrsph_ctx* c);
+// The arrays of a state saved in an adjoint blob (pos | va | ad | oid | ...),
+// read in place by the reverse steps.
+struct StateView {
+    void *pos, *va, *ad;
+    uint32_t* oid;
+};
+template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src = nullptr);
 template <class R> void enqueue_util_sort(rrsph_ctx* c) {
     Launcher L{c, nullptr};
     enqueue_sort(c, c->util_ctrl, L);
     enqueue_util_reorder<R>(c);
 }
 
-// Reorder of the current state by sorted_keys / sorted_perm into B (scratch
-// pred) and the cell-start table, for the utility queries.
-template <class R> void enqueue_util_reorder(rrsph_ctx* c) {
+// Reorder of the current state (or of the saved state src) by sorted_keys /
+// sorted_perm into B (scratch pred) and the cell-start table, for the
+// utility queries.
+template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src) {
     const uint32_t nb = grid_blocks(c->P.n, kBlock);
     if (c->P.dim == 3) {
-        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm,
-                                                         stateA3<R>(c),
+        StateA3<R> A = stateA3<R>(c);
+        if (src) A = StateA3<R>{(double4*)src->pos, (real4<R>*)src->va, (real4<R>*)src->ad, src->oid};
+        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A,
                                                          stateB3<R>(c, c->pred_scratch, c->pw_scratch, c->pres_scratch),
                                                          c->LB, c->maxslots + 2 * kMaxSlots, nullptr);
         return;
     }
     StateA<R> A = stateA<R>(c);
+    if (src) A = StateA<R>{(double2*)src->pos, (real4<R>*)src->va, (real4<R>*)src->ad, src->oid};
     StateB<R> B = stateB<R>(c, c->pred_scratch, c->pres_scratch);
     reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
                                                     c->maxslots + 2 * kMaxSlots, nullptr);
@@ -946,7 +955,7 @@ void enqueue_state_keys(rrsph_ctx* ctx) {
 template <class R>
 static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double* lam_in, double* lam_out,
                                  double* psum, const uint32_t* keys, const uint32_t* perm,
-                                 const AdjTape::Lists* lists) {
+                                 const AdjTape::Lists* lists, const StateView* src) {
     AdjTape& T = ctx->adj;
     const StepParams& P = ctx->P;
     const size_t n = P.n;
@@ -955,7 +964,7 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     if (keys) {  // the sort step k ran with, saved by the replay: reorder only
         ctx->sorted_keys = keys;
         ctx->sorted_perm = perm;
-        enqueue_util_reorder<R>(ctx);
+        enqueue_util_reorder<R>(ctx, src);
     } else {
         enqueue_util_sort<R>(ctx);
     }
@@ -1815,6 +1824,13 @@ static CopyPlan state_plan(rrsph_ctx* ctx, void* blob, bool save) {
 
 static rrsph_status save_state(rrsph_ctx* ctx, void* dst) { return enqueue_copies(ctx, state_plan(ctx, dst, true)); }
 
+// The arrays of the state saved in blob, in place (layout of state_plan).
+static StateView state_view(const rrsph_ctx* ctx, void* blob) {
+    const size_t n = ctx->P.n, pb = pos_bytes(ctx), vb = real4_bytes(ctx);
+    char* d = (char*)blob;
+    return StateView{d, d + pb * n, d + (pb + vb) * n, (uint32_t*)(d + (pb + 2 * vb) * n)};
+}
+
 static rrsph_status restore_state(rrsph_ctx* ctx, const void* src, bool keys = true) {
     if (const rrsph_status st = enqueue_copies(ctx, state_plan(ctx, const_cast<void*>(src), false)); st != RRSPH_OK)
         return st;
@@ -2149,21 +2165,25 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     return RRSPH_OK;
 }
 
-// dL/dx of the loss term of the gauges of state A (lam += ...)
-static void enqueue_gauge_grad(rrsph_ctx* ctx, uint32_t k, double* lam) {
+// dL/dx of the loss term of the gauges of state A, or of the saved state
+// src (lam += ...)
+static void enqueue_gauge_grad(rrsph_ctx* ctx, uint32_t k, double* lam, const StateView* src = nullptr) {
     AdjTape& T = ctx->adj;
     const uint32_t nb = grid_blocks(ctx->P.n, kBlock);
     const size_t o = (size_t)k * kMaxGauges;
+    const void* pos = src ? src->pos : (const void*)ctx->posA;
+    const void* va = src ? src->va : ctx->vaA;
+    const uint32_t* oid = src ? src->oid : ctx->oidA;
     if (ctx->P.dim == 3)
-        smooth_gauge_grad3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, ctx->oidA, T.spec,
+        smooth_gauge_grad3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)pos, oid, T.spec,
                                                            T.eta_bar + o, T.zs + o, T.ws + o, lam);
     else if (ctx->fp32)
-        smooth_gauge_grad<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const float4*)ctx->vaA,
-                                                                 ctx->oidA, T.spec.g, T.eta_bar + o, T.zs + o,
+        smooth_gauge_grad<float><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double2*)pos, (const float4*)va,
+                                                                 oid, T.spec.g, T.eta_bar + o, T.zs + o,
                                                                  T.ws + o, lam);
     else
-        smooth_gauge_grad<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA,
-                                                                  ctx->oidA, T.spec.g, T.eta_bar + o, T.zs + o,
+        smooth_gauge_grad<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double2*)pos, (const double4*)va,
+                                                                  oid, T.spec.g, T.eta_bar + o, T.zs + o,
                                                                   T.ws + o, lam);
 }
 
@@ -2214,19 +2234,27 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
             // list steps: the build whose lists step k read, its state in that build's slot order
             const int32_t b = T.lists ? T.seg_build[slot] : -1;
             const AdjTape::Lists* lists = b >= 0 ? &T.seg_lists[(size_t)b] : nullptr;
-            st = restore_state(ctx, (char*)T.seg + slot * T.blob, !sorted && !lists);
-            if (st != RRSPH_OK) return st;
+            // a step reordered with a saved sort reads its state where it was
+            // saved; one that sorts again needs it in A (and its cell keys)
+            char* blob = (char*)T.seg + slot * T.blob;
+            const bool in_place = sorted || lists;
+            const StateView view = state_view(ctx, blob);
+            const StateView* src = in_place ? &view : nullptr;
+            if (!in_place) {
+                st = restore_state(ctx, blob, true);
+                if (st != RRSPH_OK) return st;
+            }
             // sort and reorder state k exactly as its step did (predict with its dt)
             adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
             const uint32_t* sk = sorted ? T.seg_keys + slot * n : (lists ? lists->keys : nullptr);
             const uint32_t* sp = sorted ? T.seg_perm + slot * n : (lists ? T.iota : nullptr);
             if (ctx->fp32)
                 enqueue_reverse_step<float>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                            T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists);
+                                            T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists, src);
             else
                 enqueue_reverse_step<double>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
-                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists);
-            if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur]);  // loss term of state k
+                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists, src);
+            if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur], src);  // loss term of state k
             HIP_TRY(hipGetLastError());
             cur = 1 - cur;
         }
