@@ -153,7 +153,8 @@ __global__ void __launch_bounds__(kBlock) state_keys3_kernel(StepParams P, const
 template <class R>
 __global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, StateB3<R> B,
                                                     const double* __restrict__ lam_in, AdjRec3<R>* __restrict__ rec,
-                                                    double* __restrict__ direct, double* __restrict__ piston_sum) {
+                                                    double* __restrict__ direct, double* __restrict__ piston_sum,
+                                                    const uint32_t* __restrict__ home, float4* __restrict__ nearf) {
     __shared__ double s_p[kMaxSegments][kBlock / kWave];
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     double pist = 0.0;
@@ -214,6 +215,7 @@ __global__ void __launch_bounds__(kBlock) adj_prep3(StepParams P, AdjStep T, Sta
             r.u = B.pred[s];
         }
         rec[s] = r;
+        if (nearf) nearf[s] = near3<float>(P, ps, (float)t, home[s]);  // type in w
         double* d = direct + kDirect3 * (size_t)s;
         d[0] = rx;
         d[1] = ry;
@@ -317,11 +319,21 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
                                                     const uint32_t* __restrict__ nbc,
                                                     const AdjRec3<R>* __restrict__ rec,
                                                     const double* __restrict__ direct, double* __restrict__ lam_out,
-                                                    uint32_t* __restrict__ overflow_flag) {
+                                                    uint32_t* __restrict__ overflow_flag,
+                                                    const float4* __restrict__ nearf) {
+    // FP32 step over lists: entries decided and hits' r_i - r_k taken from
+    // the near records (16 bytes, as the list pass does) instead of the FP64
+    // positions (32 bytes)
+    constexpr bool kNear = LISTS && sizeof(R) == 4;
     __shared__ uint32_t hits[kAdjChunk * kBlock];
+    __shared__ float4 s_off[kNear ? kCodes3 : 1];
     // XCD-aware: each XCD walks one contiguous slice of the slots
     const uint32_t lb = xcd_block(P.blocks_per_xcd);
     if (lb >= P.nblocks) return;
+    if constexpr (kNear) {
+        fill_cell_offsets3(P, s_off);
+        __syncthreads();
+    }
     const uint32_t s = lb * kBlock + threadIdx.x;
     const bool live = s < P.n;
     const uint32_t sc = live ? s : P.n - 1;
@@ -348,6 +360,8 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     const double Pk = B.pw[sc];                // p / R^2
     const double pres_k = B.pres[sc];
     const AdjHit3 hk = adj_hit3(rec[sc], pk);
+    float4 nk{};
+    if constexpr (kNear) nk = nearf[sc];
     const double4 oak = hk.oa;
     const double4 oxk = hk.ox;
     const int tk = (int)pk.w;
@@ -361,24 +375,35 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
     auto flush = [&](uint32_t nh) {
         if (nh == 0) return;
         const uint32_t last = nh - 1;
+        // kNear: the hit list holds whole entries (slot and home-cell code)
         uint32_t in = my[0];
-        AdjRec3<R> rn = rec[in];
+        AdjRec3<R> rn = rec[kNear ? in & kSlotMask3 : in];
         double4 pn{};
-        if constexpr (sizeof(R) == 4) pn = B.pos[in];
+        float4 qn{};
+        if constexpr (kNear) qn = nearf[in & kSlotMask3];
+        else if constexpr (sizeof(R) == 4) pn = B.pos[in];
 #pragma unroll 1
         for (uint32_t u = 0; u < nh; ++u) {
             const AdjHit3 h = adj_hit3(rn, pn);
-            const double4 pi = h.pos;
             const double4 vai = h.v;
             const double4 pri = h.u;
             const double Pi = h.pw;
             const double4 oai = h.oa;
             const double4 oxi = h.ox;
+            int ti;
+            V3 e;  // r_i - r_k
+            if constexpr (kNear) {
+                const float4 o = s_off[in >> kSlotBits3];
+                e = V3{(double)((qn.x - nk.x) + o.x), (double)((qn.y - nk.y) + o.y), (double)((qn.z - nk.z) + o.z)};
+                ti = (int)qn.w;
+            } else {
+                e = V3{h.pos.x - pk.x, h.pos.y - pk.y, h.pos.z - pk.z};
+                ti = (int)h.pos.w;
+            }
             in = my[min(u + 1, last) * kBlock];
-            rn = rec[in];
-            if constexpr (sizeof(R) == 4) pn = B.pos[in];
-            const int ti = (int)pi.w;
-            const V3 e{pi.x - pk.x, pi.y - pk.y, pi.z - pk.z};  // r_i - r_k
+            rn = rec[kNear ? in & kSlotMask3 : in];
+            if constexpr (kNear) qn = nearf[in & kSlotMask3];
+            else if constexpr (sizeof(R) == 4) pn = B.pos[in];
             AdjPair Q;
             Q.d2 = dot3(e, e);
             Q.rinv = Q.d2 > 0.0 ? fast_rsqrt<double>(Q.d2) : 0.0;
@@ -438,7 +463,10 @@ __global__ void __launch_bounds__(kBlock) adj_pair3(StepParams P, AdjStep T, Sta
             Pb += coefb;
         }
     };
-    if (LISTS)
+    if constexpr (kNear)
+        collect_near3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, s_off, nk, nearf, pk, B.pos, nbl, nbc, my,
+                                 overflow, flush);
+    else if (LISTS)
         collect_listed3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, pk, B.pos, nbl, nbc, my, overflow, flush);
     else
         collect_neighbours3<kAdjChunk>(P, live && tk != 0, P.max_nbr - 1, sc, skey[sc], pk, B.pos, LB, my,

@@ -325,6 +325,54 @@ __device__ __forceinline__ bool list_cut3(const StepParams& P, const CutFP32&, c
     return dist2_exact3(dx, dy, dz) < P.cut2;
 }
 
+// collect_listed3 from the near records of the list steps (position
+// relative to the home cell of the build, and type, in nearf): each entry
+// decided as the FP32 list pass decides it (list_cut3: FP32, the FP64 test
+// on pos inside the band), the same hits; the hit list keeps the whole
+// entry (slot and home-cell offset code).
+template <uint32_t kChunk, class F>
+__device__ __forceinline__ void collect_near3(const StepParams& P, bool active, uint32_t cap, uint32_t s,
+                                              const float4* s_off, const float4& nk, const float4* __restrict__ nearf,
+                                              const double4& pj, const double4* __restrict__ pos,
+                                              const uint32_t* __restrict__ nbl, const uint32_t* __restrict__ nbc,
+                                              uint32_t* __restrict__ hits, bool& overflow, F&& flush) {
+    static_assert(kChunk >= 2 * kCollectL3, "hit list too short");
+    const CutFP32 C = cut_band(P);
+    const uint32_t cnt = active ? nbc[s] : 0u;
+    const uint32_t cmax = wave_max_u32(cnt);
+    const uint32_t row = P.n * 4u;
+    uint32_t nh = 0, done = 0;
+#pragma unroll 1
+    for (uint32_t k = 0; k < cmax; k += kCollectL3) {
+        uint32_t e[kCollectL3];
+        float4 q[kCollectL3];
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) e[u] = list_entry(nbl, (k + u) * row + s * 4u);
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = nearf[e[u] & kSlotMask3];
+#pragma unroll
+        for (uint32_t u = 0; u < kCollectL3; ++u) {
+            float dx, dy, dz;
+            const bool in = list_cut3(P, C, s_off, e[u], q[u], nk, pj, pos, dx, dy, dz);
+            const bool hit = (k + u < cnt) & (q[u].w != 0.f) & in;
+            if (hit) {
+                if (done + nh < cap) {
+                    hits[nh * kBlock] = e[u];
+                    ++nh;
+                } else {
+                    overflow = true;
+                }
+            }
+        }
+        if (__any(nh + kCollectL3 > kChunk)) {  // wave-uniform
+            flush(nh);
+            done += nh;
+            nh = 0;
+        }
+    }
+    flush(nh);
+}
+
 // Summation density from the lists (Density.h:25-61) with the
 // renormalisation of the steps that ask for it (ModelParams.tex:124-133):
 // rho into rec.pr.w (sign-typed), p / rho^2 into rec.nr.w, p into pres.

@@ -146,6 +146,7 @@ struct AdjTape {
     uint32_t nseg_lists = 0;             // builds of the segment in hand
     std::vector<int32_t> seg_build;      // per seg slot: the build (index in seg_lists) its step read
     uint32_t* iota = nullptr;            // identity permutation
+    float4* nearf = nullptr;             // FP32 step: near records of the reverse step's state (n)
     uint64_t force_build = ~0ull;        // tape step that must build (after a stale-list halt)
 };
 
@@ -911,6 +912,7 @@ void free_tape(AdjTape& T) {
     }
     T.seg_lists.clear();
     if (T.iota) (void)hipFree(T.iota);
+    if (T.nearf) (void)hipFree(T.nearf);
     void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
                     T.drho_adj, T.out_adj, T.flag, T.rec, T.seg_keys, T.seg_perm};
     for (void* p : ptrs)
@@ -971,13 +973,19 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     if (P.dim == 3) {
         StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
         AdjRec3<R>* rec = (AdjRec3<R>*)T.out_adj;
-        adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, rec, T.direct, psum);
+        // FP32 step over lists: near records relative to the home cells of
+        // the build (its sorted keys) for the pair adjoint's walk
+        float4* nearf = lists && sizeof(R) == 4 ? T.nearf : nullptr;
+        adj_prep3<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, rec, T.direct, psum,
+                                                     nearf ? lists->keys : nullptr, nearf);
         if (lists)
             adj_pair3<R, true><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, lists->nbl,
-                                                                   lists->nbc, rec, T.direct, lam_out, T.flag);
+                                                                   lists->nbc, rec, T.direct, lam_out, T.flag,
+                                                                   nearf);
         else
             adj_pair3<R, false><<<nb_xcd, kBlock, 0, ctx->stream>>>(P, S, B, ctx->sorted_keys, ctx->LB, nullptr,
-                                                                    nullptr, rec, T.direct, lam_out, T.flag);
+                                                                    nullptr, rec, T.direct, lam_out, T.flag,
+                                                                    nullptr);
     } else {
         StateB<R> B = stateB<R>(ctx, ctx->pred_scratch, ctx->pres_scratch);
         adj_prep<R><<<nb, kBlock, 0, ctx->stream>>>(P, S, B, lam_in, T.out_adj, T.drho_adj, T.direct, psum);
@@ -2121,6 +2129,7 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     T.nseg_lists = 0;
     T.force_build = ~0ull;
     if (T.lists) T.iota = dalloc<uint32_t>(n, e);
+    if (T.lists && ctx->fp32) T.nearf = dalloc<float4>(n, e);
     T.eta = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.zs = dalloc<double>((size_t)nsteps * kMaxGauges, e);
     T.ws = dalloc<double>((size_t)nsteps * kMaxGauges, e);
