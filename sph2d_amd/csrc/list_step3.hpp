@@ -73,14 +73,16 @@ __device__ __forceinline__ real4<R> predict3(const StepParams& P, const real4<R>
 }
 
 // The reorder of a 3D build step: reorder3_kernel's gather, pressure and
-// cell-start table, plus the records and each slot's home cell.
+// cell-start table, plus the records, each slot's home cell and its
+// candidate record for the build's walk, cand = (position relative to the
+// home cell's origin in FP32, the home cell's z index).
 template <class R>
 __global__ void __launch_bounds__(kBlock) reorder_list3_kernel(StepParams P, const Ctrl* __restrict__ ctrl,
                                                                const uint32_t* __restrict__ skey,
                                                                const uint32_t* __restrict__ perm, StateA3<R> A,
                                                                StateB3<R> B, ListRec<R>* __restrict__ rec,
                                                                real4<R>* __restrict__ dens, uint32_t* __restrict__ home,
-                                                               uint32_t* __restrict__ LB,
+                                                               float4* __restrict__ cand, uint32_t* __restrict__ LB,
                                                                unsigned long long* __restrict__ maxslots) {
     if (ctrl->halted) return;
     if (blockIdx.x == 0) {
@@ -110,6 +112,9 @@ __global__ void __launch_bounds__(kBlock) reorder_list3_kernel(StepParams P, con
         rec[s] = r;
         dens[s] = near3<R>(P, pos, typed_rho<R>(va.w, itype), k);
         home[s] = k;
+        const double3 o = cell_origin3(P, k);
+        cand[s] = make_float4((float)(pos.x - o.x), (float)(pos.y - o.y), (float)(pos.z - o.z),
+                              __int_as_float((int)(k % (uint32_t)P.ncz)));
     }
     fill_cell_starts(P, s, live, k, skey, LB);
 }
@@ -140,6 +145,7 @@ __device__ __forceinline__ int wave_max_i32(int v) {
 }
 __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl* __restrict__ ctrl,
                                                              const double4* __restrict__ pos,
+                                                             const float4* __restrict__ cand,
                                                              const uint32_t* __restrict__ skey,
                                                              const uint32_t* __restrict__ LB,
                                                              uint32_t* __restrict__ nbl, uint32_t* __restrict__ nbc) {
@@ -189,15 +195,25 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
     const int wx0 = wave_min_i32(alive ? x0 : INT_MAX), wx1 = wave_max_i32(alive ? x1 : INT_MIN);
     const int wy0 = wave_min_i32(alive ? y0 : INT_MAX), wy1 = wave_max_i32(alive ? y1 : INT_MIN);
     const double reach2 = reach * reach;
+    // candidates are tested in FP32 from the candidate records: r_i - r_s =
+    // (xi_i - xi_s) + (home-cell offset) cell, rounded far below the 1e-5
+    // by which the FP32 reach exceeds list_reach, so the lists hold every
+    // pair within list_reach (and at most a few just beyond, which every
+    // list step tests again)
+    const float4 cj = cand[s];
+    const float cellf = (float)P.cell;
+    const float r2f = (float)(P.list_r2 * (1.0 + 1e-5));
 #pragma unroll 1
     for (int x = wx0; x <= wx1; ++x) {
         // distance from the particle to the column's x slab
         const double xl = P.xmin + x * P.cell;
         const double dxm = fmax(0.0, fmax(xl - pj.x, pj.x - (xl + P.cell)));
+        const float ox = (float)(x - hx) * cellf;
 #pragma unroll 1
         for (int y = wy0; y <= wy1; ++y) {
             const double yl = P.ymin + y * P.cell;
             const double dym = fmax(0.0, fmax(yl - pj.y, pj.y - (yl + P.cell)));
+            const float oy = (float)(y - hy) * cellf;
             // the z cells of this column within reach: a column off the
             // particle's own needs a shorter range (corner columns most)
             const double rem = reach2 - dxm * dxm - dym * dym;
@@ -215,18 +231,16 @@ __global__ void __launch_bounds__(kBlock) list_build3_kernel(StepParams P, Ctrl*
             const uint32_t lmax = wave_max_u32(len);
 #pragma unroll 1
             for (uint32_t i = 0; i < lmax; i += kCollectL3) {
-                double4 q[kCollectL3];
+                float4 q[kCollectL3];
 #pragma unroll
-                for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = pos[len ? b + min(i + u, len - 1) : s];
+                for (uint32_t u = 0; u < kCollectL3; ++u) q[u] = cand[len ? b + min(i + u, len - 1) : s];
 #pragma unroll
                 for (uint32_t u = 0; u < kCollectL3; ++u) {
-                    const double dx = q[u].x - pj.x, dy = q[u].y - pj.y, dz = q[u].z - pj.z;
-                    const bool in = (i + u < len) & (b + i + u != s) & (dx * dx + dy * dy + dz * dz < P.list_r2);
+                    const int dcz = __float_as_int(q[u].w) - hz;  // the neighbour's z cell, from its key
+                    const float dx = (q[u].x - cj.x) + ox, dy = (q[u].y - cj.y) + oy;
+                    const float dz = (q[u].z - cj.z) + (float)dcz * cellf;
+                    const bool in = (i + u < len) & (b + i + u != s) & (dx * dx + dy * dy + dz * dz < r2f);
                     if (in) {
-                        // the z cell of the neighbour: its key's, from the same position by the
-                        // same rule (cell_key3), without loading the key
-                        uint32_t zbad = 0;
-                        const int dcz = cell_coord(q[u].z, P.zmin, P.cell, P.ncz, zbad) - hz;
                         const uint32_t code = code_xy + 25u * (uint32_t)(dcz + 2);
                         my[nh * kBlock] = (b + i + u) | (code << kSlotBits3);
                         ++nh;

@@ -224,6 +224,7 @@ struct rrsph_ctx {
     void* dens = nullptr;               // 3D: records of the summation density pass (real4 per slot)
     uint32_t* home = nullptr;           // each slot's cell at the last build
     unsigned long long* dslots = nullptr;  // per-step largest displacement (block maxima)
+    float4* cand = nullptr;                 // 3D builds: candidate records (FP32 home-relative position, z cell)
     hipGraph_t graph_list = nullptr, graph_build = nullptr;
     hipGraphExec_t gexec_list = nullptr, gexec_build = nullptr;
     uint64_t build_at = 0;              // step index of the last build launched
@@ -494,11 +495,11 @@ template <class R> void enqueue_list_step3_t(rrsph_ctx* c, Profiler* prof, bool 
         StateB3<R> B = stateB3<R>(c, c->pred, c->pw, c->pres);
         L(K_REORDER, [&] {
             reorder_list3_kernel<R><<<nb, kBlock, 0, c->stream>>>(P, c->ctrl, c->sorted_keys, c->sorted_perm, A, B,
-                                                                  rec, dens, c->home, c->LB, c->maxslots);
+                                                                  rec, dens, c->home, c->cand, c->LB, c->maxslots);
         });
         L(K_LIST_BUILD, [&] {
-            list_build3_kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, (const double4*)c->posB, c->sorted_keys,
-                                                                 c->LB, c->nbl, c->nbc);
+            list_build3_kernel<<<nb_xcd, kBlock, 0, c->stream>>>(P, c->ctrl, (const double4*)c->posB, c->cand,
+                                                                 c->sorted_keys, c->LB, c->nbl, c->nbc);
         });
         X = ListState3<R>{(const double4*)c->posB, (const real4<R>*)c->vaB, c->oidB};
     }
@@ -929,7 +930,7 @@ void free_all(rrsph_ctx* c) {
                     c->maxslots, c->LB, c->ctrl,
                     c->util_ctrl, c->red, c->st_r, c->st_v, c->st_rho, c->st_p, c->st_a, c->st_drho, c->st_av,
                     c->st_mu, c->st_itype, c->st_u32, c->skeyA, c->chg, c->chg_mask, c->wpre, c->pw,
-                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->rec, c->dens, c->home, c->dslots};
+                    c->pw_scratch, c->adB, c->nbl, c->nbc, c->rec, c->dens, c->home, c->dslots, c->cand};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (c->host_ctrl) (void)hipHostFree(c->host_ctrl);
@@ -1062,6 +1063,7 @@ rrsph_status rrsph_hip_create(const rrsph_params_c* params, rrsph_ctx** out) {
         ctx->nbc = dalloc<uint32_t>(n, e);
         ctx->rec = dalloc<ListRec<double>>(n, e);
         if (dim == 3) ctx->dens = dalloc<double4>(n, e);
+        if (dim == 3) ctx->cand = dalloc<float4>(n, e);
         ctx->home = dalloc<uint32_t>(n, e);
         ctx->dslots = dalloc<unsigned long long>(2 * kMaxSlots, e);
     }
