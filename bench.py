@@ -408,7 +408,7 @@ def c1_full(steps):
         states[name]["a"] = g.fields()["a"]
         states[name]["step"] = g.time()[2]
         out[name] = {"ms_per_step": round(wall / steps * 1e3, 5), "value": round(prm.ntotal * steps / wall, 1),
-                     "time_s": g.time()[0], "lists": g.list_stats(), "sort_stats": g.sort_stats(),
+                     "simulated_time_s": g.time()[0], "lists": g.list_stats(), "sort_stats": g.sort_stats(),
                      "diag": g.diagnostics()}
         g.close()
     out["fp32_over_fp64"] = round(out["fp64"]["ms_per_step"] / out["fp32"]["ms_per_step"], 3)
