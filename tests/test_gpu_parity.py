@@ -284,6 +284,24 @@ def test_incremental_resort_equals_full_sort(monkeypatch, precision):
         assert np.array_equal(full[k], tiny[k]), k
 
 
+def test_incremental_resort_above_16384_wavefronts_equals_full_sort(monkeypatch):
+    """Above 16,384 wavefronts (n > 1,048,576) the incremental re-sort counts
+    the changed slots of its ballot masks in several passes: 30 sorted steps
+    of a 1.1M-particle flume (jittered start, so particles change cell every
+    step) are bit-identical to the full radix sort's."""
+    lay = G.piston_tank(delta=0.003, jitter=1e-4)
+    prm = lay.params
+    print("incremental re-sort, flume n", prm.ntotal, flush=True)
+    assert prm.ntotal > 16384 * 64
+    monkeypatch.setenv("RRSPH_HIP_LIST", "0")
+    full, sf = _run_with_env(monkeypatch, {"RRSPH_HIP_SORT": "full"}, prm, lay.state, 30, H.FP32)
+    inc, si = _run_with_env(monkeypatch, {}, prm, lay.state, 30, H.FP32)
+    print("sort stats", sf, si, flush=True)
+    assert sf["incremental"] == 0 and si["incremental"] >= 20 and si["last_changed"] > 0
+    for k in ("r", "v", "rho", "p", "itype"):
+        assert np.array_equal(full[k], inc[k]), k
+
+
 def test_unstaged_resort_placement_equals_full_sort(monkeypatch):
     """resort_scatter stages the changed composites that fall inside a
     block's key range in LDS when there are at most 256 of them, and places
