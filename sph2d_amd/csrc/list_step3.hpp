@@ -419,24 +419,49 @@ __global__ void __launch_bounds__(kBlock) density_sum_list3(StepParams P, Ctrl* 
         const uint32_t row = P.n * 4u;
         uint32_t off = s * 4u;
         uint32_t hits = 0;
-#pragma unroll 2
-        for (uint32_t k = 0; k < cnt; ++k, off += row) {
-            const uint32_t e = list_entry(nbl, off);
-            const real4<R> di = dens[e & kSlotMask3];
-            R dx, dy, dz;
-            const bool hit = list_cut3(P, C, s_off, e, di, dj, pj, X.pos, dx, dy, dz) && di.w != R(0);
-            const bool take = hit & (hits < cap);
-            overflow |= hit & !take;
-            hits += take ? 1u : 0u;
-            const R q = pair_dist<R>(dx, dy, dz) * inv_h;
-            const R t = R(2) - q;
-            const R w = q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q) : w_q2 * t * t * t;
-            const R wt = take ? w : R(0);
-            rho += m * wt;
-            if (renorm) wsum += m / fabs(di.w) * wt;
+        // groups of G entries: the group's records gathered together, the
+        // next group's entries loaded behind them (as the pair pass does);
+        // the sums run in list order
+        constexpr uint32_t G = 4;
+        static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
+        uint32_t e[G];
+#pragma unroll
+        for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, off + u * row);
+        off += G * row;
+#pragma unroll 1
+        for (uint32_t k = 0; k < cnt; k += G) {
+            real4<R> di[G];
+            uint32_t en[G];
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) di[u] = dens[e[u] & kSlotMask3];
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, off + u * row);
+            off += G * row;
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) {
+                R dx, dy, dz;
+                const bool hit =
+                    list_cut3(P, C, s_off, e[u], di[u], dj, pj, X.pos, dx, dy, dz) & (di[u].w != R(0)) & (k + u < cnt);
+                const bool take = hit & (hits < cap);
+                overflow |= hit & !take;
+                hits += take ? 1u : 0u;
+                const R q = pair_dist<R>(dx, dy, dz) * inv_h;
+                const R t = R(2) - q;
+                const R w = q <= R(1) ? w_q1 * (R(2) / R(3) - q * q + R(0.5) * q * q * q) : w_q2 * t * t * t;
+                const R wt = take ? w : R(0);
+                rho += m * wt;
+                // (a neighbour that died since the build has density 0 in its
+                // record: only taken entries enter the sum)
+                if (renorm && take) wsum += m / fabs(di[u].w) * wt;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < G; ++u) e[u] = en[u];
         }
     }
-    if (renorm) rho = rho / wsum;
+    // a particle that died since the build has no sums to renormalise (its
+    // record's density is 0): it keeps the plain sum, so that its records,
+    // which the lists of its former neighbours still name, stay finite
+    if (renorm && wsum > R(0)) rho = rho / wsum;
     if (overflow) atomicOr(&ctrl->nbr_overflow, 1u);
     const int itype = (int)X.pos[s].w;
     const R p = eos_p<R>(rho, (R)P.rho0, (R)P.eos_B);

@@ -62,3 +62,35 @@ def test_renormalised_summation_3d(monkeypatch, lists, precision):
         assert rho_err < 1e-12 and np.max(np.abs(g["r"] - r["r"])) < 1e-13
     else:
         assert rho_err < 1e-5
+
+
+@pytest.mark.parametrize("lists", ["1", "0"])
+def test_renormalised_summation_3d_after_a_wall_disappears(monkeypatch, lists):
+    """The small 3D dam break renormalised every 2nd step, the wall behind
+    the column (x <= 0, moved to the end of the arrays as the wave maker's
+    range) disappearing (NWM_METHOD_WALL_DISAPPEAR) at step 31, when builds
+    have become rare (the fluid barely moves yet): the lists built before
+    keep the vanished particles, whose records then hold density 0, until
+    the next build; the renormalisation must skip them.  40 FP64 steps
+    against the oracle."""
+    monkeypatch.setenv("RRSPH_HIP_LIST", lists)
+    lay = G.dam_break_3d(delta=0.04)
+    prm = copy.copy(lay.params)
+    back = (lay.itype == P.TYPE_BOUNDARY) & (lay.r[:, 0] <= 1e-9)
+    order = np.concatenate([np.nonzero(~back)[0], np.nonzero(back)[0]])
+    lay = G.Layout(prm, lay.r[order], lay.v[order], lay.rho[order], lay.itype[order])
+    prm.density_normalization = 1
+    prm.density_normalization_every = 2
+    prm.nwm = P.NWM_METHOD_WALL_DISAPPEAR
+    prm.boundary_motion = True
+    prm.nwm_particles_start = prm.ntotal - int(back.sum())
+    prm.nwm_particles_end = prm.ntotal
+    prm.nwm_time_start = 30.5 * prm.dt
+    P.post_fill_in_model_params(prm)
+    g, r = _compare(lay, prm, 40, H.FP64)
+    assert np.all(g["itype"][prm.nwm_particles_start:] == 0)
+    assert np.array_equal(g["itype"], r["itype"])
+    alive = r["itype"] != 0  # the densities of non-existent particles are not compared (as for FIX)
+    assert np.all(np.isfinite(g["rho"][alive]))
+    assert np.max(np.abs(g["rho"][alive] - r["rho"][alive]) / r["rho"][alive]) < 1e-10
+    assert np.max(np.abs(g["r"] - r["r"])) < 1e-12
