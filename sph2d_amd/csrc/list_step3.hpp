@@ -417,17 +417,24 @@ __global__ void __launch_bounds__(kBlock) density_sum_list3(StepParams P, Ctrl* 
         const double4 pj = X.pos[s];
         const uint32_t cap = P.max_nbr - 1;
         const uint32_t row = P.n * 4u;
-        uint32_t off = s * 4u;
         uint32_t hits = 0;
         // groups of G entries: the group's records gathered together, the
         // next group's entries loaded behind them (as the pair pass does);
-        // the sums run in list order
-        constexpr uint32_t G = 4;
-        static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
+        // the sums run in list order.  Rows past a list's end name valid
+        // slots (padding, or entries of an earlier build); the loads past
+        // the last allocated row read that row instead.
+        // 6.6M dam break: FP32 1.13 ms per pass at G = 8 (1.45 at G = 4),
+        // FP64 2.06 ms at G = 4 with incremental offsets (2.21 clamped)
+        constexpr bool kWide = sizeof(R) == 4;
+        constexpr uint32_t G = kWide ? 8 : 4;
+        static_assert(kWide || 2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
+        const uint32_t last = P.list_cap + kListPad - 1;
+        auto entry = [&](uint32_t k) {
+            return list_entry(nbl, (kWide ? min(k, last) : k) * row + s * 4u);
+        };
         uint32_t e[G];
 #pragma unroll
-        for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, off + u * row);
-        off += G * row;
+        for (uint32_t u = 0; u < G; ++u) e[u] = entry(u);
 #pragma unroll 1
         for (uint32_t k = 0; k < cnt; k += G) {
             real4<R> di[G];
@@ -435,8 +442,7 @@ __global__ void __launch_bounds__(kBlock) density_sum_list3(StepParams P, Ctrl* 
 #pragma unroll
             for (uint32_t u = 0; u < G; ++u) di[u] = dens[e[u] & kSlotMask3];
 #pragma unroll
-            for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, off + u * row);
-            off += G * row;
+            for (uint32_t u = 0; u < G; ++u) en[u] = entry(k + G + u);
 #pragma unroll
             for (uint32_t u = 0; u < G; ++u) {
                 R dx, dy, dz;
