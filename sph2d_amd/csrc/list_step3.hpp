@@ -572,18 +572,20 @@ __global__ void __launch_bounds__(kBlock) pair_list3_kernel(StepParams P, Ctrl* 
     const uint32_t cnt = live && tj != 0 ? nbc[s] : 0u;
     const uint32_t cap = P.max_nbr - 1;
     if (cnt) {
-        constexpr uint32_t G = sizeof(R) == 4 ? 4 : 2;  // FP32 6.6M dam break: 2.00 ms per pass against 2.39 (2)
-        static_assert(2 * G - 1 <= kListPad, "the entries loaded past a list's end lie in its padding rows");
+        // FP32 6.6M dam break: 1.87 ms per pass at G = 8 (168 VGPRs, 3 waves
+        // per SIMD), 2.00 at 4, 2.39 at 2.  Rows past a list's end name
+        // valid slots (padding, or entries of an earlier build); the loads
+        // past the last allocated row read that row instead.
+        constexpr uint32_t G = sizeof(R) == 4 ? 8 : 2;
+        const uint32_t lastrow = P.list_cap + kListPad - 1;
         const bool capped = __ballot(cnt > cap) != 0;  // some list of the wave may hold more than cap neighbours
         const CutFP32 C = cut_band(P);
         const uint32_t row = P.n * 4u;
-        uint32_t off = s * 4u;
         uint32_t hits = 0;
         bool over = false;
         uint32_t e[G];
 #pragma unroll
-        for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, off + u * row);
-        off += G * row;
+        for (uint32_t u = 0; u < G; ++u) e[u] = list_entry(nbl, min(u, lastrow) * row + s * 4u);
 #pragma unroll 1
         for (uint32_t k = 0; k < cnt; k += G) {
             ListRec<R> r[G];
@@ -595,8 +597,7 @@ __global__ void __launch_bounds__(kBlock) pair_list3_kernel(StepParams P, Ctrl* 
                 if (K.cont) va[u] = X.va[e[u] & kSlotMask3];
             }
 #pragma unroll
-            for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, off + u * row);
-            off += G * row;
+            for (uint32_t u = 0; u < G; ++u) en[u] = list_entry(nbl, min(k + G + u, lastrow) * row + s * 4u);
 #pragma unroll
             for (uint32_t u = 0; u < G; ++u) {
                 R dx, dy, dz;
