@@ -233,7 +233,8 @@ rrsph_status rrsph_hip_probe(rrsph_ctx* ctx, const double* points, uint32_t npoi
 /* No reference counterpart: the reference has no derivative of its step.
  * The adjoint differentiates the step, 2D or 3D, in FP64 arithmetic on the
  * trajectory of the context's precision (an FP32 context tapes and replays
- * the FP32 step) (continuity density,
+ * the FP32 step; in 3D its pair adjoint reads the neighbours' adjoint values
+ * rounded to FP32 and their separations from FP32 records) (continuity density,
  * cubic kernels, pressure, artificial viscosity, Lennard-Jones walls, XSPH,
  * boundary motion with the Fourier piston, nwm = 6, one piston or paddles
  * in 3D) with dt independent of the state (dt_correction_method 0, or 2
