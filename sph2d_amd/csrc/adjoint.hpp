@@ -140,14 +140,23 @@ __global__ void __launch_bounds__(kBlock) smooth_gauge_partials(uint32_t n, cons
 
 // One block per gauge: each thread sums a fixed stride of the block
 // partials, then a fixed shuffle tree and wave order (deterministic);
-// writes eta[g], Z[g] = sum w e and Wsum[g] = sum w.
+// writes eta[g], Z[g] = sum w e and Wsum[g] = sum w.  With ctrl, the three
+// arrays are a tape of nsteps rows of kMaxGauges and the row is the step
+// index of the step just launched - base, as adj_record takes it.
 __global__ void __launch_bounds__(kBlock) smooth_gauge_final(uint32_t nblocks, GaugeSpec G,
                                                              const double* __restrict__ partials,
                                                              double* __restrict__ eta, double* __restrict__ zsum,
-                                                             double* __restrict__ wsum) {
+                                                             double* __restrict__ wsum, const Ctrl* __restrict__ ctrl,
+                                                             uint64_t base, uint32_t nsteps) {
     __shared__ double s_z[kBlock / kWave], s_w[kBlock / kWave];
     const uint32_t g = blockIdx.x;
     if (g >= G.n) return;
+    if (ctrl) {
+        const size_t row = (size_t)min(ctrl->step - base, (uint64_t)nsteps - 1) * kMaxGauges;
+        eta += row;
+        zsum += row;
+        wsum += row;
+    }
     const double* pz = partials + (size_t)(2 * g) * nblocks;
     const double* pw = partials + (size_t)(2 * g + 1) * nblocks;
     double z = 0, ww = 0;
@@ -236,8 +245,14 @@ struct AdjStep {
 };
 
 // After a step: what its adjoint needs from the control block.
-__global__ void adj_record(const Ctrl* __restrict__ ctrl, AdjStep* __restrict__ rec) {
+// Record of the step just launched, at rec[k], k = its step index (the
+// control block's step count, committed at the next prologue) - base,
+// clamped to the tape's nsteps records: a step after a halt does not run,
+// and its record lands on the halted step's, which the resumed range
+// writes again.
+__global__ void adj_record(const Ctrl* __restrict__ ctrl, AdjStep* __restrict__ rec, uint64_t base, uint32_t nsteps) {
     const Ctrl& c = *ctrl;
+    const uint64_t k = min(c.step - base, (uint64_t)nsteps - 1);
     AdjStep r;
     r.t = c.cur_time;
     r.dt = c.dt_new;
@@ -246,7 +261,7 @@ __global__ void adj_record(const Ctrl* __restrict__ ctrl, AdjStep* __restrict__ 
     r.piston = c.piston_active;
     r.halted = c.halted;
     r.pad = 0;
-    *rec = r;
+    rec[k] = r;
 }
 
 // The control block of a utility sort that predicts with dt_pred.

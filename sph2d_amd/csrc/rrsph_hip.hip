@@ -148,6 +148,12 @@ struct AdjTape {
     uint32_t* iota = nullptr;            // identity permutation
     float4* nearf = nullptr;             // FP32 step: near records of the reverse step's state (n)
     uint64_t force_build = ~0ull;        // tape step that must build (after a stale-list halt)
+    // the forward's sorted step with the incremental re-sort, its record and
+    // its gauges in one graph (rows from the control block's step count
+    // - tbase); captured on the forward's first such step
+    hipGraph_t tgraph = nullptr;
+    hipGraphExec_t texec = nullptr;
+    uint64_t tbase = 0;
 };
 
 struct rrsph_ctx {
@@ -903,6 +909,8 @@ rrsph_status setup_params(rrsph_ctx* ctx, const rrsph_params_c& p) {
 }
 
 void free_tape(AdjTape& T) {
+    if (T.texec) (void)hipGraphExecDestroy(T.texec);
+    if (T.tgraph) (void)hipGraphDestroy(T.tgraph);
     for (void* p : T.ck)
         if (p) (void)hipFree(p);
     T.ck.clear();
@@ -1775,8 +1783,10 @@ static rrsph_status to_gauge_spec(rrsph_ctx* ctx, const rrsph_gauge_spec* in, Ga
 }
 
 // smooth gauges of state A into eta/zs/ws (device, kMaxGauges entries each)
+// (with ctrl: into row ctrl->step - base of tapes of nsteps rows)
 static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, double* partials, double* eta,
-                                          double* zs, double* ws) {
+                                          double* zs, double* ws, const Ctrl* ctrl = nullptr, uint64_t base = 0,
+                                          uint32_t nsteps = 0) {
     const uint32_t nb = gauge_blocks(ctx->P.n);  // partials holds 2 kMaxGauges nb doubles
     if (ctx->P.dim == 3)
         smooth_gauge_partials3<<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, (const double4*)ctx->posA, G, partials);
@@ -1786,7 +1796,8 @@ static rrsph_status enqueue_smooth_gauges(rrsph_ctx* ctx, const GaugeSpec3& G, d
     else
         smooth_gauge_partials<double><<<nb, kBlock, 0, ctx->stream>>>(ctx->P.n, ctx->posA, (const double4*)ctx->vaA,
                                                                        G.g, partials);
-    smooth_gauge_final<<<std::max(G.g.n, 1u), kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws);
+    smooth_gauge_final<<<std::max(G.g.n, 1u), kBlock, 0, ctx->stream>>>(nb, G.g, partials, eta, zs, ws, ctrl, base,
+                                                                         nsteps);
     HIP_TRY(hipGetLastError());
     return RRSPH_OK;
 }
@@ -1862,10 +1873,57 @@ static rrsph_status restore_state_sync(rrsph_ctx* ctx, const void* src) {
     return RRSPH_OK;
 }
 
-// Launch steps [k0, k1) of the tape without host synchronisation: every
-// step's record, the checkpoints (forward) or segment states (replay; slot
-// k - kslot of seg_base), and the gauges (forward).  Returns the status read
-// after the last step.
+// One forward tape step as the graph of a sorted step with the incremental
+// re-sort, then its record and gauges (rows of the step index - base),
+// when that is what launch_steps would launch; returns false (nothing
+// launched) otherwise.  A launch after a kernel launched on its own (the
+// record, the gauges) waits several microseconds longer for the stream
+// than a kernel inside a graph, so the step costs about 20 us less.
+static rrsph_status launch_taped_step(rrsph_ctx* ctx, uint64_t base, bool& launched) {
+    AdjTape& T = ctx->adj;
+    launched = false;
+    if (!ctx->use_graph || ctx->need_full || !ctx->incremental || ctx->launched < ctx->full_until) return RRSPH_OK;
+    if (!T.texec || T.tbase != base) {
+        if (T.texec) (void)hipGraphExecDestroy(T.texec);
+        if (T.tgraph) (void)hipGraphDestroy(T.tgraph);
+        T.texec = nullptr;
+        T.tgraph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(ctx, nullptr, false);
+        adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec, base, T.nsteps);
+        const rrsph_status st = enqueue_smooth_gauges(ctx, T.spec, T.partials, T.eta, T.zs, T.ws, ctx->ctrl, base,
+                                                      T.nsteps);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != RRSPH_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return st != RRSPH_OK ? st : fail(ctx, RRSPH_ERR_HIP, std::string("hipStreamEndCapture: ") +
+                                                                      hipGetErrorString(e));
+        }
+        T.tgraph = g;
+        HIP_TRY(hipGraphInstantiate(&T.texec, T.tgraph, nullptr, nullptr, 0));
+        T.tbase = base;
+    }
+    HIP_TRY(hipGraphLaunch(T.texec, ctx->stream));
+    // launch_steps' bookkeeping of a step with the incremental re-sort
+    ctx->last_keys = ctx->kbuf[0];
+    ctx->last_perm = ctx->vbuf[0];
+    ++ctx->n_incremental;
+    ++ctx->launched;
+    ctx->need_full = false;
+    ctx->stepped = true;
+    ctx->fields_valid = true;
+    launched = true;
+    return RRSPH_OK;
+}
+
+constexpr uint32_t kHaltCheck = 64;  // tape steps between two looks at the control block
+
+// Launch steps [k0, k1) of the tape, reading the control block back every
+// kHaltCheck steps only: every step's record, the checkpoints (forward) or
+// segment states (replay; slot k - kslot of seg_base), and the gauges
+// (forward).  Returns the status read after the last step (or after the
+// look that found a halt).
 static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool forward, void* seg_base,
                                 uint32_t kslot, uint64_t abs0) {
     AdjTape& T = ctx->adj;
@@ -1873,8 +1931,17 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
     // the forward keeps the states of the last segment: the reverse pass
     // starts there and needs no replay of it
     const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
+    const uint64_t base = abs0 - k0;  // step index of tape step 0
     for (uint32_t k = k0; k < k1; ++k) {
         rrsph_status st;
+        // after a halt the rest of the range is launches that do nothing:
+        // look every kHaltCheck steps, so that a halt early in a long range
+        // costs at most that many (the caller resumes from the halted step)
+        if (k > k0 && (k - k0) % kHaltCheck == 0) {
+            HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (ctx->host_ctrl->halted) return sync_status(ctx);
+        }
         if (forward && k % T.every == 0) {
             if (T.lists && k > 0) {  // the build interval from the displacements so far
                 HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
@@ -1896,7 +1963,7 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
                 break;
             }
         }
-        bool build = false;
+        bool build = false, taped = false;
         if (T.lists) {
             const uint64_t abs = abs0 + (k - k0);
             build = forward ? (k % T.every == 0 || k == T.force_build || abs - ctx->build_at >= ctx->list_interval)
@@ -1904,7 +1971,8 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
             if (forward) T.builds[k] = build ? 1 : 0;
             st = launch_list_step(ctx, build, abs);
         } else {
-            st = launch_steps(ctx, 1);
+            st = forward ? launch_taped_step(ctx, base, taped) : RRSPH_OK;
+            if (st == RRSPH_OK && !taped) st = launch_steps(ctx, 1);
         }
         if (st != RRSPH_OK) return st;
         if (keep) {
@@ -1940,10 +2008,9 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
             st = enqueue_copies(ctx, plan);
             if (st != RRSPH_OK) return st;
         }
-        if (forward) {
-            adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec + k);
-            st = enqueue_smooth_gauges(ctx, T.spec, T.partials, T.eta + (size_t)k * kMaxGauges,
-                                       T.zs + (size_t)k * kMaxGauges, T.ws + (size_t)k * kMaxGauges);
+        if (forward && !taped) {
+            adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec, base, T.nsteps);
+            st = enqueue_smooth_gauges(ctx, T.spec, T.partials, T.eta, T.zs, T.ws, ctx->ctrl, base, T.nsteps);
             if (st != RRSPH_OK) return st;
         }
     }

@@ -306,3 +306,39 @@ def test_autograd_backward_after_another_forward_uses_its_own_tape():
         assert np.allclose(tb.grad.numpy(), g["b"], rtol=1e-12, atol=0)
     assert not np.allclose(ta0.grad.numpy(), ta1.grad.numpy())
     run.sim.close()
+
+
+@pytest.mark.parametrize("cap", [None, "4"])
+def test_taped_steps_in_one_graph_equal_the_launches_one_by_one(monkeypatch, cap):
+    """With graphs on, each taped forward step with the incremental re-sort
+    runs as one graph (the step, its record and its gauges, which take
+    their rows from the control block's step count); with graphs off the
+    same kernels are launched one by one.  The gauge histories are equal bit
+    for bit and so are the gradients of the initial state (the piston
+    gradients sum the wave maker's adjoints with atomics: 1e-12).  With room
+    for 4 changed slots in the re-sort, steps halt and resume with the full
+    sort, their records and gauges written again."""
+    if cap:
+        monkeypatch.setenv("RRSPH_HIP_RESORT_CAP", cap)
+    res = []
+    for graph in (True, False):
+        lay = G.adjoint_tank(delta=DELTA, length=1.2, depth=0.3, wall_height=0.6, nharm=4, sigma=0.01, seed=5)
+        prm = lay.params
+        sim = H.HipStep(prm, precision=H.FP64, graph=graph)
+        spec = H.gauge_spec([0.08, 0.16, 0.3], radius=3 * DELTA, tau=DELTA / 4, y_ref=0.3)
+        run = DifferentiableRun(sim, dict(lay.state), 60, 16, spec, prm.piston_omega)
+        a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+        eta = run.forward(a, b)
+        grads = run.backward(_weights(eta.shape), len(a))
+        res.append((eta, grads, sim.sort_stats()))
+        sim.close()
+    (e1, g1, s1), (e0, g0, s0) = res
+    print("sort stats", s1, s0)
+    assert s1["incremental"] > 0
+    if cap:
+        assert s1["fallbacks"] > 0
+    assert np.array_equal(e1, e0)
+    for k in ("r", "v", "rho"):
+        assert np.array_equal(g1[k], g0[k]), k
+    for k in ("a", "b"):
+        assert np.allclose(g1[k], g0[k], rtol=1e-12, atol=1e-12 * np.abs(g0[k]).max()), k
