@@ -264,13 +264,6 @@ __global__ void adj_record(const Ctrl* __restrict__ ctrl, AdjStep* __restrict__ 
     rec[k] = r;
 }
 
-// The control block of a utility sort that predicts with dt_pred.
-__global__ void adj_util_ctrl(Ctrl* __restrict__ uc, const AdjStep* __restrict__ rec) {
-    Ctrl c{};
-    c.cur_dt = rec->dt_pred;
-    *uc = c;
-}
-
 // whole_step reverse.  out_adj[s] = (a_x, a_y, av_x, av_y) adjoints,
 // drho_adj[s]; direct[s] = (r_x, r_y, v_x, v_y, rho) adjoints that do not
 // pass through the pair sums; piston_sum += wave-maker velocity adjoint.

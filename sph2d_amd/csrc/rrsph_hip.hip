@@ -125,6 +125,8 @@ struct AdjTape {
     uint32_t *seg_keys = nullptr, *seg_perm = nullptr;
     std::vector<uint8_t> seg_sorted;     // per seg slot: keys / perm hold that step's sort
     AdjStep* rec = nullptr;              // per step (device)
+    Ctrl* uctrl = nullptr;               // per step: the control block its reverse step sorts and reorders with
+                                         // (cur_dt = the step's dt_pred, nothing halted)
     std::vector<AdjStep> hrec;           // per step (host copy)
     double *eta = nullptr, *zs = nullptr, *ws = nullptr;  // nsteps x kMaxGauges
     double *eta_bar = nullptr, *partials = nullptr, *piston_sum = nullptr;
@@ -593,22 +595,24 @@ struct StateView {
     void *pos, *va, *ad;
     uint32_t* oid;
 };
-template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src = nullptr);
-template <class R> void enqueue_util_sort(rrsph_ctx* c) {
+template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src = nullptr, Ctrl* uc = nullptr);
+template <class R> void enqueue_util_sort(rrsph_ctx* c, Ctrl* uc = nullptr) {
     Launcher L{c, nullptr};
-    enqueue_sort(c, c->util_ctrl, L);
-    enqueue_util_reorder<R>(c);
+    enqueue_sort(c, uc ? uc : c->util_ctrl, L);
+    enqueue_util_reorder<R>(c, nullptr, uc);
 }
 
 // Reorder of the current state (or of the saved state src) by sorted_keys /
 // sorted_perm into B (scratch pred) and the cell-start table, for the
-// utility queries.
-template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src) {
+// utility queries; with the control block uc (its cur_dt predicts) instead
+// of the context's utility one.
+template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src, Ctrl* uc) {
     const uint32_t nb = grid_blocks(c->P.n, kBlock);
+    const Ctrl* ctrl = uc ? uc : c->util_ctrl;
     if (c->P.dim == 3) {
         StateA3<R> A = stateA3<R>(c);
         if (src) A = StateA3<R>{(double4*)src->pos, (real4<R>*)src->va, (real4<R>*)src->ad, src->oid};
-        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A,
+        reorder3_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, ctrl, c->sorted_keys, c->sorted_perm, A,
                                                          stateB3<R>(c, c->pred_scratch, c->pw_scratch, c->pres_scratch),
                                                          c->LB, c->maxslots + 2 * kMaxSlots, nullptr);
         return;
@@ -616,7 +620,7 @@ template <class R> void enqueue_util_reorder(rrsph_ctx* c, const StateView* src)
     StateA<R> A = stateA<R>(c);
     if (src) A = StateA<R>{(double2*)src->pos, (real4<R>*)src->va, (real4<R>*)src->ad, src->oid};
     StateB<R> B = stateB<R>(c, c->pred_scratch, c->pres_scratch);
-    reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, c->util_ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
+    reorder_kernel<R><<<nb, kBlock, 0, c->stream>>>(c->P, ctrl, c->sorted_keys, c->sorted_perm, A, B, c->LB,
                                                     c->maxslots + 2 * kMaxSlots, nullptr);
 }
 
@@ -923,7 +927,7 @@ void free_tape(AdjTape& T) {
     if (T.iota) (void)hipFree(T.iota);
     if (T.nearf) (void)hipFree(T.nearf);
     void* ptrs[] = {T.seg, T.eta, T.zs, T.ws, T.eta_bar, T.partials, T.piston_sum, T.lam[0], T.lam[1], T.direct,
-                    T.drho_adj, T.out_adj, T.flag, T.rec, T.seg_keys, T.seg_perm};
+                    T.drho_adj, T.out_adj, T.flag, T.rec, T.uctrl, T.seg_keys, T.seg_perm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     T = AdjTape{};
@@ -964,7 +968,7 @@ void enqueue_state_keys(rrsph_ctx* ctx) {
 // The reverse of one step on its re-sorted state (state precision R, adjoint
 // arithmetic FP64): whole_step reverse, then the pair gather.
 template <class R>
-static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double* lam_in, double* lam_out,
+static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, Ctrl* uc, const double* lam_in, double* lam_out,
                                  double* psum, const uint32_t* keys, const uint32_t* perm,
                                  const AdjTape::Lists* lists, const StateView* src) {
     AdjTape& T = ctx->adj;
@@ -975,9 +979,9 @@ static void enqueue_reverse_step(rrsph_ctx* ctx, const AdjStep& S, const double*
     if (keys) {  // the sort step k ran with, saved by the replay: reorder only
         ctx->sorted_keys = keys;
         ctx->sorted_perm = perm;
-        enqueue_util_reorder<R>(ctx, src);
+        enqueue_util_reorder<R>(ctx, src, uc);
     } else {
-        enqueue_util_sort<R>(ctx);
+        enqueue_util_sort<R>(ctx, uc);
     }
     if (P.dim == 3) {
         StateB3<R> B = stateB3<R>(ctx, ctx->pred_scratch, ctx->pw_scratch, ctx->pres_scratch);
@@ -1932,12 +1936,32 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
     // starts there and needs no replay of it
     const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
     const uint64_t base = abs0 - k0;  // step index of tape step 0
+    // the copies after a step (its sort, its build's lists) wait for the
+    // state copy before the next step and go in the same launch when the
+    // ranges fit one; the next step's sort overwrites them only after that
+    CopyPlan pending;
+    auto flush = [&]() -> rrsph_status {
+        if (pending.n == 0) return RRSPH_OK;
+        const CopyPlan p = pending;
+        pending.n = 0;
+        return enqueue_copies(ctx, p);
+    };
+    auto save = [&](void* dst) -> rrsph_status {
+        const CopyPlan p = state_plan(ctx, dst, true);
+        if (pending.n + p.n > kCopyRanges) {
+            if (const rrsph_status s2 = flush(); s2 != RRSPH_OK) return s2;
+        }
+        for (int i = 0; i < p.n; ++i) pending.r[pending.n++] = p.r[i];
+        return flush();
+    };
     for (uint32_t k = k0; k < k1; ++k) {
         rrsph_status st;
         // after a halt the rest of the range is launches that do nothing:
         // look every kHaltCheck steps, so that a halt early in a long range
         // costs at most that many (the caller resumes from the halted step)
         if (k > k0 && (k - k0) % kHaltCheck == 0) {
+            st = flush();
+            if (st != RRSPH_OK) return st;
             HIP_TRY(hipMemcpyAsync(ctx->host_ctrl, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
             HIP_TRY(hipStreamSynchronize(ctx->stream));
             if (ctx->host_ctrl->halted) return sync_status(ctx);
@@ -1948,13 +1972,13 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
                 HIP_TRY(hipStreamSynchronize(ctx->stream));
                 if (!ctx->host_ctrl->halted) adapt_list_interval(ctx);
             }
-            st = save_state(ctx, T.ck[k / T.every]);
+            st = save(T.ck[k / T.every]);
             if (st != RRSPH_OK) return st;
         }
         const bool keep = forward ? k >= last0 : true;  // seg holds this step's state
         const size_t slot = forward ? k - last0 : k - kslot;
         if (keep) {
-            st = save_state(ctx, (char*)seg_base + slot * T.blob);
+            st = save((char*)seg_base + slot * T.blob);
             if (st != RRSPH_OK) return st;
             if (!forward && k + 1 == k1) {  // the state before the segment's last step is all the adjoint needs
                 // that step is not launched: a list step reads the lists of the segment's last build; a
@@ -2005,8 +2029,7 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
                 ++T.nseg_lists;
             }
             if (T.lists) T.seg_build[slot] = (int32_t)T.nseg_lists - 1;
-            st = enqueue_copies(ctx, plan);
-            if (st != RRSPH_OK) return st;
+            pending = plan;  // empty before: the state copy before the step took it
         }
         if (forward && !taped) {
             adj_record<<<1, 1, 0, ctx->stream>>>(ctx->ctrl, T.rec, base, T.nsteps);
@@ -2014,6 +2037,7 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
             if (st != RRSPH_OK) return st;
         }
     }
+    if (const rrsph_status st = flush(); st != RRSPH_OK) return st;
     return sync_status(ctx);
 }
 
@@ -2216,6 +2240,7 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
         return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
     }
     T.rec = dalloc<AdjStep>(nsteps, e);
+    T.uctrl = dalloc<Ctrl>(nsteps, e);
     if (e != hipSuccess) {
         free_tape(ctx->adj);
         return fail(ctx, RRSPH_ERR_HIP, std::string("hipMalloc (adjoint tape): ") + hipGetErrorString(e));
@@ -2239,6 +2264,12 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < nsteps; ++k)
         for (uint32_t g = 0; g < G.g.n; ++g) eta[(size_t)k * G.g.n + g] = h[(size_t)k * kMaxGauges + g];
+    {
+        std::vector<Ctrl> uc(nsteps, Ctrl{});
+        for (uint32_t k = 0; k < nsteps; ++k) uc[k].cur_dt = T.hrec[k].dt_pred;
+        HIP_TRY(hipMemcpyAsync(T.uctrl, uc.data(), nsteps * sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     T.valid = true;
     return RRSPH_OK;
 }
@@ -2323,14 +2354,13 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
                 if (st != RRSPH_OK) return st;
             }
             // sort and reorder state k exactly as its step did (predict with its dt)
-            adj_util_ctrl<<<1, 1, 0, ctx->stream>>>(ctx->util_ctrl, T.rec + k);
             const uint32_t* sk = sorted ? T.seg_keys + slot * n : (lists ? lists->keys : nullptr);
             const uint32_t* sp = sorted ? T.seg_perm + slot * n : (lists ? T.iota : nullptr);
             if (ctx->fp32)
-                enqueue_reverse_step<float>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
+                enqueue_reverse_step<float>(ctx, T.hrec[k], T.uctrl + k, T.lam[cur], T.lam[1 - cur],
                                             T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists, src);
             else
-                enqueue_reverse_step<double>(ctx, T.hrec[k], T.lam[cur], T.lam[1 - cur],
+                enqueue_reverse_step<double>(ctx, T.hrec[k], T.uctrl + k, T.lam[cur], T.lam[1 - cur],
                                              T.piston_sum + (size_t)k * kMaxSegments, sk, sp, lists, src);
             if (k >= 1) enqueue_gauge_grad(ctx, k - 1, T.lam[1 - cur], src);  // loss term of state k
             HIP_TRY(hipGetLastError());
