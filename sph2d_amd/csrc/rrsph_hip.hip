@@ -118,8 +118,12 @@ struct AdjTape {
     GaugeSpec3 spec{};
     size_t blob = 0;                     // bytes of one saved state
     std::vector<void*> ck;               // checkpoint before step k * every; last entry: final state
-    void* seg = nullptr;                 // states of one replayed segment (every blobs)
+    void* seg = nullptr;                 // states of one replayed segment (every blobs), or of every step
     bool seg_last = false;               // seg holds the last segment's states (kept by the forward)
+    // seg holds the state before every step of the run (and the sorts):
+    // the forward keeps them all when they fit in half the free device
+    // memory (tapes of sorted steps), and the reverse pass replays nothing
+    bool keep_all = false;
     // the cell sort each step of seg ran with (keys, permutation): the reverse
     // step reorders its state with them instead of sorting it again
     uint32_t *seg_keys = nullptr, *seg_perm = nullptr;
@@ -1932,9 +1936,9 @@ static rrsph_status adjoint_run(rrsph_ctx* ctx, uint32_t k0, uint32_t k1, bool f
                                 uint32_t kslot, uint64_t abs0) {
     AdjTape& T = ctx->adj;
     const size_t n = ctx->P.n;
-    // the forward keeps the states of the last segment: the reverse pass
-    // starts there and needs no replay of it
-    const uint32_t last0 = (T.nsteps - 1) / T.every * T.every;
+    // the forward keeps the states of the last segment (of every step with
+    // keep_all): the reverse pass starts there and needs no replay of it
+    const uint32_t last0 = T.keep_all ? 0 : (T.nsteps - 1) / T.every * T.every;
     const uint64_t base = abs0 - k0;  // step index of tape step 0
     // the copies after a step (its sort, its build's lists) wait for the
     // state copy before the next step and go in the same launch when the
@@ -2212,13 +2216,24 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
     hipError_t e = hipSuccess;
     T.ck.assign(nck + 1, nullptr);
     for (auto& p : T.ck) p = dalloc<char>(T.blob, e);
-    T.seg = dalloc<char>(T.blob * T.every, e);
-    T.seg_keys = dalloc<uint32_t>(n * T.every, e);
-    T.seg_perm = dalloc<uint32_t>(n * T.every, e);
-    T.seg_sorted.assign(T.every, 0);
     T.lists = list_active(ctx) && d3;
+    {
+        // every step's state and sort when they fit in half the free memory
+        // (RRSPH_HIP_ADJ_STORE=segments keeps one segment's, and replays)
+        size_t free_b = 0, total_b = 0;
+        const char* store = std::getenv("RRSPH_HIP_ADJ_STORE");
+        const bool segments = store && std::strcmp(store, "segments") == 0;
+        const double all_b = (double)nsteps * (double)(T.blob + 2 * n * sizeof(uint32_t));
+        T.keep_all = !T.lists && !segments && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+                     all_b <= 0.5 * (double)free_b;
+    }
+    const uint32_t cap = T.keep_all ? nsteps : T.every;  // states seg holds
+    T.seg = dalloc<char>(T.blob * cap, e);
+    T.seg_keys = dalloc<uint32_t>(n * cap, e);
+    T.seg_perm = dalloc<uint32_t>(n * cap, e);
+    T.seg_sorted.assign(cap, 0);
     T.builds.assign(nsteps, 0);
-    T.seg_build.assign(T.every, -1);
+    T.seg_build.assign(cap, -1);
     T.nseg_lists = 0;
     T.force_build = ~0ull;
     if (T.lists) T.iota = dalloc<uint32_t>(n, e);
@@ -2328,7 +2343,7 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
         const uint32_t n0 = (uint32_t)seg * T.every, n1 = std::min(n0 + T.every, N);
         // replay the segment from its checkpoint, keeping the state before each
         // step (the last segment's states are there already after a forward)
-        if (!(T.seg_last && seg == (int)nck - 1)) {
+        if (!T.keep_all && !(T.seg_last && seg == (int)nck - 1)) {
             std::fill(T.seg_sorted.begin(), T.seg_sorted.end(), (uint8_t)0);
             std::fill(T.seg_build.begin(), T.seg_build.end(), -1);
             st = restore_state(ctx, T.ck[seg]);
@@ -2338,7 +2353,7 @@ static rrsph_status adjoint_backward_impl(rrsph_ctx* ctx, const double* eta_bar,
         }
         T.seg_last = false;
         for (int k = (int)n1 - 1; k >= (int)n0; --k) {
-            const size_t slot = (size_t)(k - n0);
+            const size_t slot = (size_t)(T.keep_all ? k : k - (int)n0);
             const bool sorted = T.seg_sorted[slot] != 0;  // the replay launched step k with a sort
             // list steps: the build whose lists step k read, its state in that build's slot order
             const int32_t b = T.lists ? T.seg_build[slot] : -1;

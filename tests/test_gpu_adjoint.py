@@ -342,3 +342,41 @@ def test_taped_steps_in_one_graph_equal_the_launches_one_by_one(monkeypatch, cap
         assert np.array_equal(g1[k], g0[k]), k
     for k in ("a", "b"):
         assert np.allclose(g1[k], g0[k], rtol=1e-12, atol=1e-12 * np.abs(g0[k]).max()), k
+
+
+@pytest.mark.parametrize("cap", [None, "4"])
+def test_kept_states_equal_the_replayed_segments(monkeypatch, cap):
+    """When every step's state fits in half the free device memory the
+    forward keeps them all and the reverse pass replays nothing;
+    RRSPH_HIP_ADJ_STORE=segments keeps one segment's and replays the others
+    from their checkpoints.  A replay repeats its steps exactly, so the
+    reverse steps see the same states and sorts: equal gauges and state
+    gradients bit for bit (piston gradients: atomics, 1e-12), with and
+    without re-sort halts in the forward and the replays."""
+    if cap:
+        monkeypatch.setenv("RRSPH_HIP_RESORT_CAP", cap)
+    res = {}
+    for store in ("all", "segments"):
+        if store == "segments":
+            monkeypatch.setenv("RRSPH_HIP_ADJ_STORE", "segments")
+        else:
+            monkeypatch.delenv("RRSPH_HIP_ADJ_STORE", raising=False)
+        lay, run = _case(nsteps=60, every=16)
+        prm = lay.params
+        a, b = np.array(prm.piston_a), np.array(prm.piston_b)
+        eta = run.forward(a, b)
+        grads = run.backward(_weights(eta.shape), len(a))
+        after = run.sim.download()
+        res[store] = (eta, grads, after, run.sim.sort_stats())
+        run.sim.close()
+    (e1, g1, s1, q1), (e0, g0, s0, q0) = res["all"], res["segments"]
+    print("sort stats", q1, q0)
+    if cap:
+        assert q1["fallbacks"] > 0
+    assert q0["incremental"] + q0["full"] > q1["incremental"] + q1["full"]  # the replays ran
+    assert np.array_equal(e1, e0)
+    for k in ("r", "v", "rho"):
+        assert np.array_equal(g1[k], g0[k]), k
+        assert np.array_equal(s1[k], s0[k]), k  # the context ends on the forward's final state
+    for k in ("a", "b"):
+        assert np.allclose(g1[k], g0[k], rtol=1e-12, atol=1e-12 * np.abs(g0[k]).max()), k
