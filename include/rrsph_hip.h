@@ -273,16 +273,18 @@ rrsph_status rrsph_hip_smooth_gauges_noalloc(rrsph_ctx* ctx, const rrsph_gauge_s
 rrsph_status rrsph_hip_set_piston(rrsph_ctx* ctx, double omega, uint32_t nharm, const double* a, const double* b);
 /* Differentiable run from the current state: nsteps steps, a checkpoint
  * every checkpoint_every steps (device memory), the smooth gauges after
- * every step into eta[step * spec->n + g] (host). */
+ * every step into eta[step * spec->n + g] (host).  Steps that sort every
+ * step keep the state before every step instead when all of them fit in
+ * half the free device memory (then the reverse pass replays nothing). */
 rrsph_status rrsph_hip_adjoint_forward(rrsph_ctx* ctx, uint32_t nsteps, uint32_t checkpoint_every,
                                        const rrsph_gauge_spec* spec, double* eta);
 /* Reverse pass of the last adjoint_forward for eta_bar = dL/d eta (same
  * layout): gradients with respect to the piston coefficients (segments *
  * nharm each, the layout of set_piston) and, when the pointers are not
  * NULL, with respect to the initial state (original particle order; r, v as
- * dim-tuples).  Replays each checkpoint
- * segment forward and runs the step adjoints backward; leaves the state as
- * adjoint_forward left it. */
+ * dim-tuples).  Replays each checkpoint segment forward (unless the
+ * forward kept every state) and runs the step adjoints backward; leaves the
+ * state as adjoint_forward left it. */
 rrsph_status rrsph_hip_adjoint_backward(rrsph_ctx* ctx, const double* eta_bar, double* grad_a, double* grad_b,
                                         double* grad_r0, double* grad_v0, double* grad_rho0);
 
