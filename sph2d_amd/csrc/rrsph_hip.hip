@@ -2227,10 +2227,23 @@ static rrsph_status adjoint_forward_impl(rrsph_ctx* ctx, uint32_t nsteps, uint32
         T.keep_all = !T.lists && !segments && hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
                      all_b <= 0.5 * (double)free_b;
     }
-    const uint32_t cap = T.keep_all ? nsteps : T.every;  // states seg holds
+    uint32_t cap = T.keep_all ? nsteps : T.every;  // states seg holds
+    const bool alloc_ok = e == hipSuccess;
     T.seg = dalloc<char>(T.blob * cap, e);
     T.seg_keys = dalloc<uint32_t>(n * cap, e);
     T.seg_perm = dalloc<uint32_t>(n * cap, e);
+    if (alloc_ok && e != hipSuccess && T.keep_all) {  // every state did not fit after all: one segment's
+        void* got[] = {T.seg, T.seg_keys, T.seg_perm};
+        for (void* p : got)
+            if (p) (void)hipFree(p);
+        (void)hipGetLastError();
+        e = hipSuccess;
+        T.keep_all = false;
+        cap = T.every;
+        T.seg = dalloc<char>(T.blob * cap, e);
+        T.seg_keys = dalloc<uint32_t>(n * cap, e);
+        T.seg_perm = dalloc<uint32_t>(n * cap, e);
+    }
     T.seg_sorted.assign(cap, 0);
     T.builds.assign(nsteps, 0);
     T.seg_build.assign(cap, -1);
