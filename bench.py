@@ -365,7 +365,13 @@ def precision_error(s32, s64, delta):
             "a_max_rel_to_scale": da}
 
 
-def cpu_leg(layout, steps):
+def cpu_leg(layout, steps, samples=3):
+    """The FP64 OpenMP oracle on the same flume: after 2 warm-up steps,
+    `samples` consecutive runs of steps // samples steps each.  Returns the
+    median rate (particle-steps/s), the threads, the seconds of all runs and
+    the rate of each run."""
+    import statistics
+
     import oracle
 
     prm = layout.params
@@ -373,12 +379,17 @@ def cpu_leg(layout, steps):
     o = oracle.Oracle(prm, num_threads=threads)
     o.set_state(layout.r, layout.v, layout.rho, layout.itype, time=0.0, dt=prm.dt)
     o.step(2)
-    t0 = time.perf_counter()
-    st = o.step(steps)
-    t1 = time.perf_counter()
-    if st != 0:
-        raise RuntimeError(f"oracle status {st}")
-    return prm.ntotal * steps / (t1 - t0), threads, t1 - t0
+    per = max(steps // samples, 1)
+    rates, secs = [], 0.0
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        st = o.step(per)
+        t1 = time.perf_counter()
+        if st != 0:
+            raise RuntimeError(f"oracle status {st}")
+        rates.append(prm.ntotal * per / (t1 - t0))
+        secs += t1 - t0
+    return statistics.median(rates), threads, secs, rates
 
 
 def c1_full(steps):
@@ -581,14 +592,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu_rate, cores, secs = cpu_leg(layout, args.cpu_steps)
+        cpu_rate, cores, secs, rates = cpu_leg(layout, args.cpu_steps)
         cpu = {
             "value": round(cpu_rate, 1),
             "unit": "particle-steps/s",
             "cores": cores,
             "kind": "port",
-            "sample": f"{args.cpu_steps} steps of the same {n}-particle flume after 2 warm-up steps "
-                      f"({secs:.1f} s), FP64 OpenMP oracle",
+            "sample": f"median of {len(rates)} consecutive runs of {max(args.cpu_steps // len(rates), 1)} steps "
+                      f"of the same {n}-particle flume after 2 warm-up steps ({secs:.1f} s in all; runs at "
+                      + ", ".join(f"{r:.3g}" for r in rates) + " particle-steps/s), FP64 OpenMP oracle",
         }
 
     out = {
