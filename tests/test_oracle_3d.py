@@ -8,7 +8,10 @@ golden vectors (their cases are 2D): closed forms and brute force.
   (GridUtils.h:49-66, brute force over bits);
 * the 3D neighbour sets equal an O(N^2) search;
 * the pressure and viscosity pair forces conserve momentum (sum m a_i = 0
-  over a closed set of particles, gravity and walls excluded).
+  over a closed set of particles, gravity and walls excluded);
+* the renormalised summation density (docs/latex/ModelParams.tex:124-133)
+  reproduces a uniform density exactly and follows the two-particle formula
+  with the 3D kernel.
 """
 import copy
 import ctypes
@@ -119,3 +122,58 @@ def test_3d_pair_forces_conserve_momentum():
     scale = np.abs(a_int).sum()
     assert np.abs(a_int.sum(axis=0)).max() < 1e-9 * scale
     assert np.abs(a_av.sum(axis=0)).max() < 1e-9 * max(np.abs(a_av).sum(), 1e-30)
+
+
+def _renorm3(prm, r, itype, rho_old):
+    """The oracle's renormalised and plain summation densities of state r."""
+    st, nb = oracle.grid_find(prm, r, itype)
+    assert st == 0
+    op = oracle.from_params(prm)
+    n = prm.ntotal
+    rho, p, plain, pp = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    nbf = np.ascontiguousarray(nb.reshape(-1))
+    oracle.lib().oracle_density_sum_normalized(ctypes.byref(op), oracle.ptr(r), oracle.ptr(nbf, ctypes.c_uint32),
+                                               oracle.ptr(np.ascontiguousarray(rho_old, dtype=np.float64)),
+                                               oracle.ptr(rho), oracle.ptr(p))
+    oracle.lib().oracle_density_sum(ctypes.byref(op), oracle.ptr(r), oracle.ptr(nbf, ctypes.c_uint32),
+                                    oracle.ptr(plain), oracle.ptr(pp))
+    return rho, plain
+
+
+def test_3d_renormalised_density_reproduces_a_uniform_density():
+    """rho_j = sum m W / sum (m / rho_i) W is c for every rho_i = c, also at
+    the free surface, the walls and the corners of the 3D dam break, where
+    the plain summation falls short of rho0."""
+    lay, prm = _p3(1)
+    prm.density_treatment = 0
+    for c in (1000.0, 1020.5):
+        rho, plain = _renorm3(prm, lay.r, lay.itype, np.full(prm.ntotal, c))
+        assert np.max(np.abs(rho - c)) <= 4e-16 * c * 8
+    assert plain.min() < 0.8 * prm.rho0  # missing kernel support at the surface and in the corners
+
+
+def test_3d_renormalised_density_two_particle_closed_form():
+    """Two particles q h apart along a 3D direction:
+    rho_j = (m W(0) + m W(qh)) / ((m / rho_j) W(0) + (m / rho_i) W(qh)), W the 3D cubic spline."""
+    lay, prm = _p3(1)
+    prm.density_treatment = 0
+    prm.ntotal = prm.nfluid = 2
+    h = prm.hsml
+    L = oracle.lib()
+    L.oracle_kernel_w.restype = ctypes.c_double
+    L.oracle_kernel_w.argtypes = [ctypes.POINTER(oracle.OracleParams), ctypes.c_double, ctypes.c_int32]
+    op = oracle.from_params(prm)
+    u = np.array([0.48, 0.6, 0.64])  # unit vector
+    base = np.array([0.2, 0.15, 0.1])
+    for q in (0.4, 1.3, 1.9):
+        r = np.array([base, base + q * h * u])
+        rho_old = np.array([1003.0, 991.0])
+        rho, plain = _renorm3(prm, r, np.array([2, 2], dtype=np.int32), rho_old)
+        w0 = L.oracle_kernel_w(ctypes.byref(op), 0.0, 1)
+        w = L.oracle_kernel_w(ctypes.byref(op), q * h, 1)
+        m = prm.mass
+        for j, i in ((0, 1), (1, 0)):
+            want = (m * w0 + m * w) / (m / rho_old[j] * w0 + m / rho_old[i] * w)
+            assert rho[j] == pytest.approx(want, rel=1e-14)
+            assert plain[j] == pytest.approx(m * (w0 + w), rel=1e-14)
