@@ -11,7 +11,9 @@ golden vectors (their cases are 2D): closed forms and brute force.
   over a closed set of particles, gravity and walls excluded);
 * the renormalised summation density (docs/latex/ModelParams.tex:124-133)
   reproduces a uniform density exactly and follows the two-particle formula
-  with the 3D kernel.
+  with the 3D kernel;
+* the segmented Fourier piston moves each paddle's particles along x at the
+  derivative of that paddle's own displacement series.
 """
 import copy
 import ctypes
@@ -177,3 +179,43 @@ def test_3d_renormalised_density_two_particle_closed_form():
             want = (m * w0 + m * w) / (m / rho_old[j] * w0 + m / rho_old[i] * w)
             assert rho[j] == pytest.approx(want, rel=1e-14)
             assert plain[j] == pytest.approx(m * (w0 + w), rel=1e-14)
+
+
+def test_segmented_piston_moves_each_paddle_with_its_own_series():
+    """A 3D basin whose piston is split into 4 paddles across z, each with
+    its own 3 harmonics (paddle-major coefficients): after one oracle step
+    from t0, every piston particle moves along x at dX_g/dt (t0), X_g(t) =
+    sum_k a_gk sin(k w t) + b_gk cos(k w t) the displacement of its paddle g
+    (central differences of X_g), g the paddle whose z range holds the
+    particle (the end rows beyond the paddles belong to the end paddles)."""
+    segs, nh = 4, 3
+    lay = G.piston_basin_3d(delta=0.05, length=1.0, width=0.8, depth=0.2, segments=segs, nharm=nh, sigma=0.05)
+    prm = lay.params
+    a = np.array(prm.piston_a).reshape(segs, nh)
+    b = np.array(prm.piston_b).reshape(segs, nh)
+    w = prm.piston_omega
+    t0 = 0.37
+    o = oracle.Oracle(prm)
+    o.set_state(lay.r, lay.v, lay.rho, lay.itype, time=t0, dt=prm.dt)
+    assert o.step(1) == 0
+    v = o.state()["v"]
+    k = np.arange(1, nh + 1)
+
+    def X(g, t):
+        return float(np.sum(a[g] * np.sin(k * w * t) + b[g] * np.cos(k * w * t)))
+
+    eps = 1e-6
+    vel = [(X(g, t0 + eps) - X(g, t0 - eps)) / (2 * eps) for g in range(segs)]
+    assert min(abs(vel[g] - vel[h]) for g in range(segs) for h in range(g)) > 1e-3  # the paddles differ
+    i0, i1 = prm.nwm_particles_start, prm.nwm_particles_end
+    z = lay.r[i0:i1, 2]
+    width = (prm.piston_seg_zmax - prm.piston_seg_zmin) / segs
+    seen = set()
+    for j, zz in enumerate(z):
+        g = 0
+        while g + 1 < segs and zz >= prm.piston_seg_zmin + (g + 1) * width:
+            g += 1
+        seen.add(g)
+        assert v[i0 + j, 0] == pytest.approx(vel[g], rel=1e-8, abs=1e-12), (j, zz, g)
+        assert v[i0 + j, 1] == 0.0 and v[i0 + j, 2] == 0.0
+    assert seen == set(range(segs))
