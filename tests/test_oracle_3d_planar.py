@@ -10,9 +10,12 @@ tests/test_oracle_3d.py): the summation and continuity densities, the
 pressure force (both approximations), the artificial viscosity and the
 average velocity.  The external force (gravity and the Lennard-Jones wall
 force) has no kernel and is the same, as are the viscosity's mu and the
-pressures; every z component is 0.  With every z difference 0 the z parts of
-the pair terms are not exercised here: the momentum and gradient checks of
-tests/test_oracle_3d.py cover them.
+pressures; every z component is 0.  The same case lifted into the x-z
+plane instead, (x, y) -> (x, 0, y), exercises the z parts: its squared
+distances are the same sums bit for bit, so the neighbour sets are too, and
+the (x, z) components of the kernel-weighted terms equal the scaled 2D
+(x, y) ones with y exactly 0 (gravity is not rotated: the external force
+is compared whole in the x-y lift, its wall part in both).
 """
 import copy
 import ctypes
@@ -24,17 +27,24 @@ import oracle
 from helpers import load_dambreak_case
 
 
-@pytest.fixture(scope="module")
-def planar():
+@pytest.fixture(scope="module", params=["xy", "xz"])
+def planar(request):
     prm2, st, _tg = load_dambreak_case()
     prm3 = copy.copy(prm2)
     prm3.dim = 3
-    prm3.z_mingeom, prm3.z_maxgeom = -0.1, 0.1
     n = prm2.ntotal
     r2 = np.ascontiguousarray(st["r"], dtype=np.float64)
     v2 = np.ascontiguousarray(st["v"], dtype=np.float64)
-    r3 = np.ascontiguousarray(np.column_stack([r2, np.zeros(n)]))
-    v3 = np.ascontiguousarray(np.column_stack([v2, np.zeros(n)]))
+    z = np.zeros(n)
+    if request.param == "xy":
+        prm3.z_mingeom, prm3.z_maxgeom = -0.1, 0.1
+        r3 = np.ascontiguousarray(np.column_stack([r2[:, 0], r2[:, 1], z]))
+        v3 = np.ascontiguousarray(np.column_stack([v2[:, 0], v2[:, 1], z]))
+    else:
+        prm3.z_mingeom, prm3.z_maxgeom = prm2.y_mingeom, prm2.y_maxgeom
+        prm3.y_mingeom, prm3.y_maxgeom = -0.1, 0.1
+        r3 = np.ascontiguousarray(np.column_stack([r2[:, 0], z, r2[:, 1]]))
+        v3 = np.ascontiguousarray(np.column_stack([v2[:, 0], z, v2[:, 1]]))
     s2, nb2 = oracle.grid_find(prm2, r2, st["itype"])
     s3, nb3 = oracle.grid_find(prm3, r3, st["itype"])
     assert s2 == 0 and s3 == 0
@@ -48,7 +58,7 @@ def planar():
     assert ratio == pytest.approx(7.0 / (10.0 * prm2.hsml), rel=1e-6)
     c2 = (prm2, r2, v2, nb2)
     c3 = (prm3, r3, v3, nb3)
-    return c2, c3, st, ratio
+    return c2, c3, st, ratio, request.param
 
 
 def _sets(nb, n):
@@ -60,7 +70,7 @@ def _sets(nb, n):
 
 
 def test_planar_neighbour_sets_are_the_2d_sets(planar):
-    (prm2, _r2, _v2, nb2), (_p3, _r3, _v3, nb3), _st, _ratio = planar
+    (prm2, _r2, _v2, nb2), (_p3, _r3, _v3, nb3), _st, _ratio, _lift = planar
     n = prm2.ntotal
     s2, s3 = _sets(nb2, n), _sets(nb3, n)
     assert s2 == s3
@@ -106,12 +116,14 @@ def _run(case, st, which, p=None, **kw):
     raise ValueError(which)
 
 
-def _scaled(out3, out2, scale):
+def _scaled(out3, out2, scale, lift="xy"):
     """out3 = scale * out2 to 1e-12 of the largest value (the 3D walk sums
-    the same pairs in another order); the z components are exactly 0."""
+    the same pairs in another order), the 2D axes mapped by the lift; the
+    third component is exactly 0."""
     if out3.ndim == 2:
-        assert np.all(out3[:, 2] == 0.0)
-        out3 = out3[:, :2]
+        axes, zero = ((0, 1), 2) if lift == "xy" else ((0, 2), 1)
+        assert np.all(out3[:, zero] == 0.0)
+        out3 = out3[:, list(axes)]
     want = scale * out2
     top = float(np.max(np.abs(want)))
     assert top > 0.0
@@ -120,46 +132,50 @@ def _scaled(out3, out2, scale):
 
 
 def test_planar_densities(planar):
-    c2, c3, st, ratio = planar
+    c2, c3, st, ratio, lift = planar
     d2, p2 = _run(c2, st, "density_con")
     d3, p3 = _run(c3, st, "density_con")
-    _scaled(d3, d2, ratio)
+    _scaled(d3, d2, ratio, lift)
     assert np.array_equal(p3, p2)  # the EOS of the same densities
     s2, _ = _run(c2, st, "density_sum")
     s3, _ = _run(c3, st, "density_sum")
-    _scaled(s3, s2, ratio)
+    _scaled(s3, s2, ratio, lift)
 
 
 @pytest.mark.parametrize("approx", [1, 2])
 def test_planar_pressure_force(planar, approx):
-    c2, c3, st, ratio = planar
+    c2, c3, st, ratio, lift = planar
     _d, p = _run(c2, st, "density_con")
     a2 = _run(c2, st, "internal", p=p, intf_sph_approximation=approx, artificial_pressure=0)
     a3 = _run(c3, st, "internal", p=p, intf_sph_approximation=approx, artificial_pressure=0)
-    _scaled(a3, a2, ratio)
+    _scaled(a3, a2, ratio, lift)
 
 
-@pytest.mark.parametrize("sbt", [0, 1])
-def test_planar_wall_force(planar, sbt):
-    c2, c3, st, _ratio = planar
-    a2 = _run(c2, st, "external", boundary_treatment=sbt)  # gravity along -y, plus the walls' LJ force
-    a3 = _run(c3, st, "external", boundary_treatment=sbt)
-    _scaled(a3, a2, 1.0)
-    if sbt == 1:
-        assert np.any(a2[:, 0] != 0.0)  # the walls act
+def test_planar_wall_force(planar):
+    """The Lennard-Jones wall force (repulsive boundaries minus the dynamic
+    ones, which leaves gravity out) rotates with the lift; in the x-y lift
+    the whole external force, gravity along -y included, is the 2D one."""
+    c2, c3, st, _ratio, lift = planar
+    ext = {(d, sbt): _run(c, st, "external", boundary_treatment=sbt) for d, c in ((2, c2), (3, c3)) for sbt in (0, 1)}
+    lj2, lj3 = ext[2, 1] - ext[2, 0], ext[3, 1] - ext[3, 0]
+    assert np.any(lj2[:, 0] != 0.0)  # the walls act
+    _scaled(lj3, lj2, 1.0, lift)
+    if lift == "xy":
+        for sbt in (0, 1):
+            _scaled(ext[3, sbt], ext[2, sbt], 1.0)
 
 
 def test_planar_artificial_viscosity(planar):
-    c2, c3, st, ratio = planar
+    c2, c3, st, ratio, lift = planar
     a2, mu2 = _run(c2, st, "art_visc", dt_correction_method=2, artificial_viscosity=True)
     a3, mu3 = _run(c3, st, "art_visc", dt_correction_method=2, artificial_viscosity=True)
-    _scaled(a3, a2, ratio)
+    _scaled(a3, a2, ratio, lift)
     assert np.max(np.abs(mu3 - mu2)) <= 1e-12 * np.max(np.abs(mu2))
 
 
 def test_planar_average_velocity(planar):
-    c2, c3, st, ratio = planar
+    c2, c3, st, ratio, lift = planar
     kw = dict(average_velocity=True, average_velocity_skf=1, average_velocity_coef=0.05)
     a2 = _run(c2, st, "average_velocity", **kw)
     a3 = _run(c3, st, "average_velocity", **kw)
-    _scaled(a3, a2, ratio)
+    _scaled(a3, a2, ratio, lift)
