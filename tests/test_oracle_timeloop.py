@@ -257,31 +257,33 @@ def test_3d_loop_matches_composed_modules(density_treatment):
     assert np.max(np.abs(st["v"] - lay.v)) > 1e-4  # the particles moved
 
 
-def test_update_acceleration_is_the_sum_of_the_pinned_terms():
+@pytest.mark.parametrize("dim", [2, 3])
+def test_update_acceleration_is_the_sum_of_the_pinned_terms(dim):
     """update_acceleration's a = external (gravity + LJ) + internal
     (pressure) + artificial viscosity, each of which matches the reference's
-    golden vectors; its XSPH output equals the pinned average-velocity term
-    and its mu the pinned AV mu (UpdateAcceleration.cpp:59-100)."""
-    lay = G.dam_break()
+    golden vectors (in 3D: through the planar lifts of the golden case); its
+    XSPH output equals the pinned average-velocity term and its mu the
+    pinned AV mu (UpdateAcceleration.cpp:59-100)."""
+    lay = G.dam_break() if dim == 2 else G.dam_break_3d()
     prm = lay.params
     L = oracle.lib()
     op = oracle.from_params(prm)
-    n = prm.ntotal
+    n, D = prm.ntotal, prm.dim
     r, rho, it = lay.r, lay.rho, lay.itype
-    v = np.random.default_rng(1).normal(0, 0.05, (n, 2))
+    v = np.random.default_rng(1).normal(0, 0.05, (n, D))
     rho = rho * (1 + np.random.default_rng(2).normal(0, 1e-3, n))
     st, nb = oracle.grid_find(prm, r, it)
     nbf = np.ascontiguousarray(nb.reshape(-1))
     p = np.zeros(n); drho = np.zeros(n)
     L.oracle_density_con(ctypes.byref(op), oracle.ptr(r), oracle.ptr(v), oracle.ptr(nbf, ctypes.c_uint32),
                          oracle.ptr(rho), oracle.ptr(drho), oracle.ptr(p))
-    a = np.zeros((n, 2)); av = np.zeros((n, 2)); mu = np.zeros(n)
+    a = np.zeros((n, D)); av = np.zeros((n, D)); mu = np.zeros(n)
     amax, mumax = ctypes.c_double(), ctypes.c_double()
     L.oracle_update_acceleration(ctypes.byref(op), oracle.ptr(it, ctypes.c_int32), oracle.ptr(r), oracle.ptr(v),
                                  oracle.ptr(nbf, ctypes.c_uint32), oracle.ptr(rho), oracle.ptr(p), oracle.ptr(a),
                                  oracle.ptr(av), oracle.ptr(mu), ctypes.byref(amax), ctypes.byref(mumax))
-    ext = np.zeros((n, 2)); inf = np.zeros((n, 2)); arv = np.zeros((n, 2)); mu2 = np.zeros(n)
-    xs = np.zeros((n, 2))
+    ext = np.zeros((n, D)); inf = np.zeros((n, D)); arv = np.zeros((n, D)); mu2 = np.zeros(n)
+    xs = np.zeros((n, D))
     L.oracle_term_external(ctypes.byref(op), oracle.ptr(r), oracle.ptr(it, ctypes.c_int32),
                            oracle.ptr(nbf, ctypes.c_uint32), oracle.ptr(ext))
     L.oracle_term_internal(ctypes.byref(op), oracle.ptr(r), oracle.ptr(nbf, ctypes.c_uint32), oracle.ptr(p),
@@ -295,5 +297,5 @@ def test_update_acceleration_is_the_sum_of_the_pinned_terms():
     assert np.max(np.abs(a - want)) <= 1e-12 * scale
     assert np.max(np.abs(mu - mu2)) <= 1e-12 * max(np.max(mu2), 1e-30)
     assert np.max(np.abs(av - xs)) <= 1e-12 * max(np.max(np.abs(xs)), 1e-30)
-    assert amax.value == pytest.approx(np.max(np.hypot(a[:, 0], a[:, 1])), rel=1e-14)
+    assert amax.value == pytest.approx(np.max(np.linalg.norm(a, axis=1)), rel=1e-14)
     assert mumax.value == pytest.approx(np.max(mu), rel=1e-14)
