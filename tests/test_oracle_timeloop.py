@@ -81,6 +81,39 @@ def test_fix_kills_and_stop_halts():
             assert ora.outside() == 1
 
 
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_3d_fix_kills_and_stop_halts_on_every_face(axis):
+    """3D: a fluid particle leaving the box through the upper face of each
+    axis (z included): FIX turns it into itype 0 and keeps its position,
+    STOP returns the outside status, PRINT keeps going with it outside."""
+    prm = copy.copy(G.dam_break_3d().params)
+    prm.ntotal, prm.nfluid, prm.nvirt = 2, 1, 1
+    lo = np.array([prm.x_mingeom, prm.y_mingeom, prm.z_mingeom])
+    hi = np.array([prm.x_maxgeom, prm.y_maxgeom, prm.z_maxgeom])
+    mid = 0.5 * (lo + hi)
+    r = np.array([mid, lo + 0.1 * (hi - lo)])
+    r[0, axis] = hi[axis] - 1e-4
+    v = np.zeros((2, 3))
+    v[0, axis] = 10.0
+    rho = np.array([1000.0, 1000.0])
+    itype = np.array([2, -2], dtype=np.int32)
+    prm.g = 0.0  # only the push across the face moves the particle
+    for treatment, status in [(2, 0), (1, 3), (0, 0)]:
+        p = copy.copy(prm)
+        p.consistency_treatment = treatment
+        p.consistency_check = True
+        ora = oracle.Oracle(p)
+        ora.set_state(r, v, rho, itype, time=0.0, dt=p.dt)
+        assert ora.step(1) == status
+        st = ora.state()
+        if treatment == 2:
+            assert st["itype"][0] == 0 and np.array_equal(st["r"][0], r[0])
+            assert ora.outside() == 0
+        else:
+            assert st["itype"][0] == 2 and st["r"][0][axis] > hi[axis]
+            assert ora.outside() == 1
+
+
 def test_dt_rules():
     """dt_correction_method 0 keeps params.dt; method 2 in the OpenMP build's
     mode (disabled reductions) gives CFL h / c; in the reduction mode
