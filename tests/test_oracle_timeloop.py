@@ -58,6 +58,36 @@ def test_free_fall_kinematics(boundary_motion):
     assert not f["av"].any() and not f["drho"].any()
 
 
+@pytest.mark.parametrize("boundary_motion", [False, True])
+def test_3d_free_fall_kinematics(boundary_motion):
+    """The same in 3D (gravity along -y, the vertical axis of the 3D cases):
+    every component of v and r follows the leapfrog closed form."""
+    prm = copy.copy(G.dam_break_3d().params)
+    prm.ntotal, prm.nfluid, prm.nvirt = 2, 1, 1
+    prm.boundary_motion = boundary_motion
+    prm.density_treatment = 1  # continuity: with no neighbours drho = 0
+    r = np.array([[0.4, 0.3, 0.2], [0.7, 0.05, 0.35]])
+    v = np.array([[0.3, 0.1, -0.2], [0.05, 0.02, 0.04]])
+    rho = np.array([1000.0, 1000.0])
+    itype = np.array([2, -2], dtype=np.int32)
+    ora = oracle.Oracle(prm)
+    ora.set_state(r, v, rho, itype, time=0.0, dt=prm.dt)
+    assert ora.step(3) == 0
+    st = ora.state()
+    g = np.array([0.0, -prm.g, 0.0])
+    dt = prm.dt
+    vf, rf = v[0].copy(), r[0].copy()
+    for k in range(3):
+        vf = vf + g * (dt * 0.5 if k == 0 else dt)
+        rf = rf + vf * dt
+    assert np.allclose(st["v"][0], vf, rtol=0, atol=1e-15)
+    assert np.allclose(st["r"][0], rf, rtol=0, atol=1e-15)
+    want_wall = r[1] + 3 * dt * v[1] if boundary_motion else r[1]
+    assert np.allclose(st["r"][1], want_wall, rtol=0, atol=1e-15)
+    assert np.array_equal(st["v"][1], v[1]) and np.array_equal(st["rho"], rho)
+    assert np.allclose(ora.fields()["a"], np.tile(g, (2, 1)))
+
+
 def test_fix_kills_and_stop_halts():
     """A fluid particle leaving the box: FIX turns it into itype 0 and keeps
     its position (WholeStep.cpp:46-53); STOP returns the outside status
